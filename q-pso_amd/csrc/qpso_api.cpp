@@ -1,0 +1,810 @@
+// qpso_api.cpp -- C ABI of the gfx950 Q-PSO hot path (include/qpso.h).
+//
+// Host orchestration replacing the reference's estimate-pose block
+// (main.cu:1269-1733): parameters (main.cu:1303-1331), random table
+// (main.cu:1337-1352), uploads (main.cu:1366-1557), InitQPSO (main.cu:1607)
+// and the iteration loop (main.cu:1625-1727).  All device memory of a solve
+// lives in one arena; the mesh and the random table are shared by solves.
+#include "qpso.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <unordered_map>
+#include <vector>
+
+#include "qpso_internal.h"
+#include "qpso_kernels.h"
+
+namespace {
+
+constexpr int kRows = QPSO_ROWS;
+constexpr int kCols = QPSO_COLS;
+constexpr int kFrame = kRows * kCols;
+constexpr int kStaleCap = 1024;
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// ---- GSL mt19937 (main.cu:1340-1352): gsl_rng_default with seed 0 -> 4357,
+//      2002 init_genrand seeding, gsl_rng_uniform = u32 / 2^32 --------------
+struct Mt19937 {
+    uint32_t mt[624];
+    int mti;
+    explicit Mt19937(uint32_t s) {
+        if (s == 0) s = 4357u;
+        mt[0] = s;
+        for (int i = 1; i < 624; ++i) mt[i] = 1812433253u * (mt[i - 1] ^ (mt[i - 1] >> 30)) + (uint32_t)i;
+        mti = 624;
+    }
+    uint32_t next() {
+        if (mti >= 624) {
+            for (int k = 0; k < 624; ++k) {
+                const uint32_t y = (mt[k] & 0x80000000u) | (mt[(k + 1) % 624] & 0x7fffffffu);
+                mt[k] = mt[(k + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+            }
+            mti = 0;
+        }
+        uint32_t y = mt[mti++];
+        y ^= y >> 11;
+        y ^= (y << 7) & 0x9d2c5680u;
+        y ^= (y << 15) & 0xefc60000u;
+        y ^= y >> 18;
+        return y;
+    }
+};
+
+int rand_stride(int iters) { return std::max(512, 12 * (iters + 1)); }
+
+struct Mesh {
+    float* d = nullptr;
+    uint32_t F = 0;
+};
+
+struct ArenaLayout {
+    size_t desc, items, obs, pos, vel, pbest, pbest_fit, fit, gbest, iter, flags, counters;
+    size_t stale0, stale1, cnt0, cnt1, fit_tr, gbest_tr, argmin_tr, total;
+};
+
+ArenaLayout layout(int N, int I, int rw, int rh, bool trace) {
+    ArenaLayout L{};
+    size_t o = 0;
+    auto take = [&](size_t bytes) { size_t r = o; o = align_up(o + bytes, 256); return r; };
+    L.desc = take(sizeof(QpsoSolveDev));
+    L.items = take(sizeof(QpsoItem) * N);
+    L.obs = take(sizeof(uint16_t) * (size_t)rw * rh);
+    L.pos = take(sizeof(float) * 7 * N);
+    L.vel = take(sizeof(float) * 7 * N);
+    L.pbest = take(sizeof(float) * 7 * N);
+    L.pbest_fit = take(sizeof(float) * N);
+    L.fit = take(sizeof(float) * N);
+    L.gbest = take(sizeof(float) * 8);
+    L.iter = take(sizeof(int32_t));
+    L.flags = take(sizeof(int32_t));
+    L.counters = take(sizeof(unsigned long long) * 3);
+    L.stale0 = take(sizeof(QpsoStale) * (size_t)N * kStaleCap);
+    L.stale1 = take(sizeof(QpsoStale) * (size_t)N * kStaleCap);
+    L.cnt0 = take(sizeof(int32_t) * N);
+    L.cnt1 = take(sizeof(int32_t) * N);
+    if (trace) {
+        L.fit_tr = take(sizeof(float) * (size_t)I * N);
+        L.gbest_tr = take(sizeof(float) * 8 * (size_t)I);
+        L.argmin_tr = take(sizeof(int32_t) * (size_t)I);
+    }
+    L.total = o;
+    return L;
+}
+
+}  // namespace
+
+struct qpso_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;  // serialises API calls that use `stream`
+    std::string err;
+    std::map<uint32_t, Mesh> meshes;
+    uint32_t next_mesh = 1;
+    std::map<std::tuple<uint32_t, int, int>, float*> rand_cache;
+    int rf_cfg = 0;
+    int n_cu = 0;
+    // cached batch plan (graph of a whole batched run)
+    struct Plan {
+        std::vector<qpso_solve*> solves;
+        int max_iters = 0;
+        QpsoSolveDev* d_descs = nullptr;
+        QpsoItem* d_items = nullptr;
+        int n_items = 0;
+        hipGraphExec_t exec = nullptr;
+    } plan;
+};
+
+struct qpso_solve {
+    qpso_ctx* ctx = nullptr;
+    QpsoSolveDev h{};             // host copy of the device descriptor
+    QpsoSolveDev* d_desc = nullptr;
+    char* arena = nullptr;
+    ArenaLayout L{};
+    bool trace = false;
+    qpso_params params{};
+    int32_t rect[4]{};
+    uint32_t mesh_id = 0;
+};
+
+namespace {
+
+int set_err(qpso_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                   \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return set_err((ctx), QPSO_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+int validate(const qpso_cluster* cl, const qpso_params* p) {
+    if (!cl || !p || !cl->depth) return QPSO_E_INVAL;
+    if (p->n_particles <= 0 || p->n_particles > 65535 || p->n_iters <= 0) return QPSO_E_INVAL;
+    const int32_t* r = cl->rect;
+    if (r[0] < 0 || r[1] < 0 || r[2] > kCols || r[3] > kRows || r[2] <= r[0] || r[3] <= r[1])
+        return QPSO_E_INVAL;
+    if (cl->num_points == 0) return QPSO_E_INVAL;
+    if (!(p->t_bound >= 0.0f) || !(p->t_bound_z >= 0.0f)) return QPSO_E_INVAL;
+    return QPSO_OK;
+}
+
+int get_rand(qpso_ctx* ctx, uint32_t seed, int N, int S, float** out) {
+    auto key = std::make_tuple(seed, N, S);
+    auto it = ctx->rand_cache.find(key);
+    if (it != ctx->rand_cache.end()) { *out = it->second; return QPSO_OK; }
+    std::vector<float> h((size_t)S * N);
+    Mt19937 mt(seed);
+    for (auto& v : h) v = (float)((double)mt.next() / 4294967296.0);
+    float* d = nullptr;
+    HIP_TRY(ctx, hipMalloc(&d, h.size() * sizeof(float)));
+    HIP_TRY(ctx, hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    ctx->rand_cache[key] = d;
+    *out = d;
+    return QPSO_OK;
+}
+
+void invalidate_plan(qpso_ctx* ctx) {
+    auto& P = ctx->plan;
+    if (P.exec) hipGraphExecDestroy(P.exec);
+    if (P.d_descs) hipFree(P.d_descs);
+    if (P.d_items) hipFree(P.d_items);
+    P = qpso_ctx::Plan{};
+}
+
+int read_result(qpso_solve* s, qpso_result* out, hipStream_t st) {
+    qpso_ctx* ctx = s->ctx;
+    float g[8];
+    unsigned long long c[3];
+    int32_t flags = 0;
+    HIP_TRY(ctx, hipMemcpyAsync(g, s->arena + s->L.gbest, sizeof(g), hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipMemcpyAsync(c, s->arena + s->L.counters, sizeof(c), hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipMemcpyAsync(&flags, s->arena + s->L.flags, sizeof(flags), hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    if (out) {
+        std::memcpy(out->pose, g, sizeof(float) * 7);
+        out->fit = g[7];
+        out->raster_px = c[0];
+        out->visit_px = c[1];
+        out->n_degenerate = (uint32_t)c[2];
+        out->status = (flags & QPSO_FLAG_STALE_OVERFLOW) ? QPSO_E_STALE : QPSO_OK;
+    }
+    return (flags & QPSO_FLAG_STALE_OVERFLOW) ? set_err(ctx, QPSO_E_STALE, "stale-pixel list overflow")
+                                               : QPSO_OK;
+}
+
+// Launch the whole iteration loop of one solve (plain form).
+int launch_solve(qpso_solve* s, hipStream_t st) {
+    qpso_ctx* ctx = s->ctx;
+    HIP_TRY(ctx, qpso_launch::init(s->d_desc, 1, st));
+    const QpsoItem* items = reinterpret_cast<const QpsoItem*>(s->arena + s->L.items);
+    for (int it = 0; it < s->h.I; ++it) {
+        HIP_TRY(ctx, qpso_launch::render_fitness(ctx->rf_cfg, s->d_desc, items, s->h.N, st));
+        HIP_TRY(ctx, qpso_launch::pso_step(s->d_desc, 1, st));
+    }
+    return QPSO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* qpso_status_str(int status) {
+    switch (status) {
+        case QPSO_OK: return "ok";
+        case QPSO_E_INVAL: return "invalid argument";
+        case QPSO_E_HIP: return "HIP runtime error";
+        case QPSO_E_NOMEM: return "out of memory";
+        case QPSO_E_BUDGET: return "request exceeds the HBM budget";
+        case QPSO_E_NOTFOUND: return "unknown id";
+        case QPSO_E_STALE: return "stale-pixel list overflow";
+        case QPSO_E_PENDING: return "pending";
+        default: return "unknown status";
+    }
+}
+
+void qpso_params_default(qpso_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->n_particles = 1024;       // Nparticle_ (main.h:84)
+    p->n_iters = 20;             // max_steps_ (main.h:99)
+    p->w = 0.3;                  // psoW_ (main.h:105)
+    p->c1 = 1.0f;                // c1_ (main.h:103)
+    p->c2 = 1.0f;                // c2_ (main.h:104)
+    p->t_bound = 0.3f;           // main.cu:1324
+    p->t_bound_z = 0.3f;         // main.cu:1325
+    p->seed = 0;                 // gsl default seed
+    p->cam = {572.41140, 573.57043, 325.26110, 242.04899};  // FX, FY, CX, CY (main.h:30-33)
+    p->face_cap = 0;
+}
+
+int qpso_create(int device, qpso_ctx** out) {
+    if (!out) return QPSO_E_INVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0 || device < 0 || device >= n) return QPSO_E_HIP;
+    if (hipSetDevice(device) != hipSuccess) return QPSO_E_HIP;
+    auto* ctx = new qpso_ctx();
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return QPSO_E_HIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->n_cu = prop.multiProcessorCount;
+    if (const char* e = std::getenv("QPSO_RF_CFG")) {
+        const int c = std::atoi(e);
+        if (c >= 0 && c < qpso_launch::rf_num_configs()) ctx->rf_cfg = c;
+    }
+    *out = ctx;
+    return QPSO_OK;
+}
+
+int qpso_destroy(qpso_ctx* ctx) {
+    if (!ctx) return QPSO_E_INVAL;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    invalidate_plan(ctx);
+    for (auto& kv : ctx->meshes) hipFree(kv.second.d);
+    for (auto& kv : ctx->rand_cache) hipFree(kv.second);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return QPSO_OK;
+}
+
+const char* qpso_last_error(qpso_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int qpso_mesh_upload(qpso_ctx* ctx, const float* soa, uint32_t n_faces, uint32_t* mesh_id) {
+    if (!ctx || !soa || n_faces == 0 || !mesh_id || n_faces > (1u << 26)) return QPSO_E_INVAL;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    Mesh m;
+    m.F = n_faces;
+    HIP_TRY(ctx, hipMalloc(&m.d, sizeof(float) * 9 * (size_t)n_faces));
+    HIP_TRY(ctx, hipMemcpy(m.d, soa, sizeof(float) * 9 * (size_t)n_faces, hipMemcpyHostToDevice));
+    *mesh_id = ctx->next_mesh++;
+    ctx->meshes[*mesh_id] = m;
+    return QPSO_OK;
+}
+
+int qpso_mesh_release(qpso_ctx* ctx, uint32_t mesh_id) {
+    if (!ctx) return QPSO_E_INVAL;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    auto it = ctx->meshes.find(mesh_id);
+    if (it == ctx->meshes.end()) return QPSO_E_NOTFOUND;
+    hipStreamSynchronize(ctx->stream);
+    hipFree(it->second.d);
+    ctx->meshes.erase(it);
+    return QPSO_OK;
+}
+
+uint64_t qpso_solve_bytes(const qpso_params* p, const int32_t rect[4]) {
+    if (!p || !rect) return 0;
+    const int rw = std::max(0, rect[2] - rect[0]), rh = std::max(0, rect[3] - rect[1]);
+    return layout(p->n_particles, p->n_iters, rw, rh, false).total;
+}
+
+int qpso_solve_create(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl, const qpso_params* p,
+                      int with_trace, qpso_solve** out) {
+    if (!ctx || !out) return QPSO_E_INVAL;
+    *out = nullptr;
+    if (int v = validate(cl, p)) return set_err(ctx, v, "invalid cluster or parameters");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    auto mit = ctx->meshes.find(mesh_id);
+    if (mit == ctx->meshes.end()) return set_err(ctx, QPSO_E_NOTFOUND, "unknown mesh id");
+    const int N = p->n_particles, I = p->n_iters, S = rand_stride(I);
+    const int rw = cl->rect[2] - cl->rect[0], rh = cl->rect[3] - cl->rect[1];
+
+    std::unique_ptr<qpso_solve> s(new qpso_solve());
+    s->ctx = ctx;
+    s->trace = with_trace != 0;
+    s->params = *p;
+    s->mesh_id = mesh_id;
+    std::memcpy(s->rect, cl->rect, sizeof(s->rect));
+    s->L = layout(N, I, rw, rh, s->trace);
+    if (hipMalloc(&s->arena, s->L.total) != hipSuccess)
+        return set_err(ctx, QPSO_E_NOMEM, "solve arena allocation failed");
+    float* rand = nullptr;
+    if (int r = get_rand(ctx, p->seed, N, S, &rand)) { hipFree(s->arena); return r; }
+
+    QpsoSolveDev& h = s->h;
+    char* A = s->arena;
+    h.mesh = mit->second.d;
+    h.F = (int32_t)mit->second.F;
+    h.F_eff = (p->face_cap > 0 && (uint32_t)p->face_cap < mit->second.F) ? p->face_cap : (int32_t)mit->second.F;
+    h.fx = p->cam.fx; h.fy = p->cam.fy; h.cx = p->cam.cx; h.cy = p->cam.cy;
+    h.N = N; h.I = I; h.S = S;
+    h.w = p->w; h.c1 = p->c1; h.c2 = p->c2;
+    // translation bounds around the centroid in metres (main.cu:1290-1331)
+    const float cxm = cl->centroid_mm[0] * 0.001f, cym = cl->centroid_mm[1] * 0.001f,
+                czm = cl->centroid_mm[2] * 0.001f;
+    h.x_lo[0] = cxm - p->t_bound;   h.x_hi[0] = cxm + p->t_bound;
+    h.x_lo[1] = cym - p->t_bound;   h.x_hi[1] = cym + p->t_bound;
+    h.x_lo[2] = czm - p->t_bound_z; h.x_hi[2] = czm + p->t_bound_z;
+    std::memcpy(h.rect, cl->rect, sizeof(h.rect));
+    h.num_points = cl->num_points;
+    h.obs = reinterpret_cast<const uint16_t*>(A + s->L.obs);
+    h.rand = rand;
+    h.pos = reinterpret_cast<float*>(A + s->L.pos);
+    h.vel = reinterpret_cast<float*>(A + s->L.vel);
+    h.pbest = reinterpret_cast<float*>(A + s->L.pbest);
+    h.pbest_fit = reinterpret_cast<float*>(A + s->L.pbest_fit);
+    h.fit = reinterpret_cast<float*>(A + s->L.fit);
+    h.gbest = reinterpret_cast<float*>(A + s->L.gbest);
+    h.iter = reinterpret_cast<int32_t*>(A + s->L.iter);
+    h.stale[0] = reinterpret_cast<QpsoStale*>(A + s->L.stale0);
+    h.stale[1] = reinterpret_cast<QpsoStale*>(A + s->L.stale1);
+    h.stale_cnt[0] = reinterpret_cast<int32_t*>(A + s->L.cnt0);
+    h.stale_cnt[1] = reinterpret_cast<int32_t*>(A + s->L.cnt1);
+    h.stale_cap = kStaleCap;
+    h.flags = reinterpret_cast<int32_t*>(A + s->L.flags);
+    h.counters = reinterpret_cast<unsigned long long*>(A + s->L.counters);
+    if (s->trace) {
+        h.fit_trace = reinterpret_cast<float*>(A + s->L.fit_tr);
+        h.gbest_trace = reinterpret_cast<float*>(A + s->L.gbest_tr);
+        h.argmin_trace = reinterpret_cast<int32_t*>(A + s->L.argmin_tr);
+    }
+    s->d_desc = reinterpret_cast<QpsoSolveDev*>(A + s->L.desc);
+
+    // cluster depth: only its rect is non-zero, keep that crop resident
+    std::vector<uint16_t> crop((size_t)rw * rh);
+    for (int y = 0; y < rh; ++y)
+        std::memcpy(&crop[(size_t)y * rw], cl->depth + (size_t)(cl->rect[1] + y) * kCols + cl->rect[0],
+                    sizeof(uint16_t) * rw);
+    std::vector<QpsoItem> items(N);
+    for (int i = 0; i < N; ++i) items[i] = QpsoItem{0, i};
+    hipStream_t st = ctx->stream;
+    int rc = QPSO_OK;
+    if (hipMemcpyAsync(A + s->L.obs, crop.data(), crop.size() * 2, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(A + s->L.items, items.data(), items.size() * sizeof(QpsoItem), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(s->d_desc, &s->h, sizeof(QpsoSolveDev), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        rc = set_err(ctx, QPSO_E_HIP, "solve upload failed");
+    if (rc) { hipFree(s->arena); return rc; }
+    *out = s.release();
+    return QPSO_OK;
+}
+
+int qpso_solve_destroy(qpso_solve* s) {
+    if (!s) return QPSO_E_INVAL;
+    qpso_ctx* ctx = s->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    auto& P = ctx->plan;
+    if (std::find(P.solves.begin(), P.solves.end(), s) != P.solves.end()) invalidate_plan(ctx);
+    hipFree(s->arena);
+    delete s;
+    return QPSO_OK;
+}
+
+int qpso_solve_run(qpso_solve* s, qpso_result* out) {
+    if (!s) return QPSO_E_INVAL;
+    qpso_ctx* ctx = s->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (int r = launch_solve(s, ctx->stream)) return r;
+    return read_result(s, out, ctx->stream);
+}
+
+int qpso_estimate_pose(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl, const qpso_params* p,
+                       qpso_result* out) {
+    qpso_solve* s = nullptr;
+    int r = qpso_solve_create(ctx, mesh_id, cl, p, 0, &s);
+    if (r) return r;
+    r = qpso_solve_run(s, out);
+    qpso_solve_destroy(s);
+    return r;
+}
+
+int qpso_solve_get_trace(qpso_solve* s, qpso_trace* tr) {
+    if (!s || !tr) return QPSO_E_INVAL;
+    qpso_ctx* ctx = s->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const int N = s->h.N, I = s->h.I;
+    const char* A = s->arena;
+    if ((tr->fit_trace || tr->gbest_trace || tr->argmin_trace) && !s->trace)
+        return set_err(ctx, QPSO_E_INVAL, "solve was created without traces");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (tr->fit_trace) HIP_TRY(ctx, hipMemcpy(tr->fit_trace, A + s->L.fit_tr, sizeof(float) * (size_t)I * N, hipMemcpyDeviceToHost));
+    if (tr->gbest_trace) HIP_TRY(ctx, hipMemcpy(tr->gbest_trace, A + s->L.gbest_tr, sizeof(float) * 8 * (size_t)I, hipMemcpyDeviceToHost));
+    if (tr->argmin_trace) HIP_TRY(ctx, hipMemcpy(tr->argmin_trace, A + s->L.argmin_tr, sizeof(int32_t) * (size_t)I, hipMemcpyDeviceToHost));
+    if (tr->pos) HIP_TRY(ctx, hipMemcpy(tr->pos, A + s->L.pos, sizeof(float) * 7 * N, hipMemcpyDeviceToHost));
+    if (tr->vel) HIP_TRY(ctx, hipMemcpy(tr->vel, A + s->L.vel, sizeof(float) * 7 * N, hipMemcpyDeviceToHost));
+    if (tr->pbest) HIP_TRY(ctx, hipMemcpy(tr->pbest, A + s->L.pbest, sizeof(float) * 7 * N, hipMemcpyDeviceToHost));
+    if (tr->pbest_fit) HIP_TRY(ctx, hipMemcpy(tr->pbest_fit, A + s->L.pbest_fit, sizeof(float) * N, hipMemcpyDeviceToHost));
+    if (tr->frames) {
+        // rebuild the reference's per-particle buffers after the last fitness pass:
+        // empty except for the carried (never visited) pixels
+        int32_t it = 0;
+        HIP_TRY(ctx, hipMemcpy(&it, A + s->L.iter, sizeof(int32_t), hipMemcpyDeviceToHost));
+        const int par = it & 1;
+        std::vector<int32_t> cnt(N);
+        std::vector<QpsoStale> ent((size_t)N * kStaleCap);
+        HIP_TRY(ctx, hipMemcpy(cnt.data(), A + (par ? s->L.cnt1 : s->L.cnt0), sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+        HIP_TRY(ctx, hipMemcpy(ent.data(), A + (par ? s->L.stale1 : s->L.stale0), sizeof(QpsoStale) * ent.size(), hipMemcpyDeviceToHost));
+        for (int p = 0; p < N; ++p) {
+            int32_t* fr = tr->frames + (size_t)p * kFrame;
+            std::fill(fr, fr + kFrame, (int32_t)QPSO_EMPTY);
+            for (int k = 0; k < cnt[p]; ++k) {
+                const QpsoStale e = ent[(size_t)p * kStaleCap + k];
+                fr[(e.pos >> 16) * kCols + (e.pos & 0xffff)] = e.val;
+            }
+        }
+    }
+    return QPSO_OK;
+}
+
+int qpso_solve_eval_poses(qpso_solve* s, int n, const float* poses, float* fit, int32_t* aabb,
+                          int32_t* frames) {
+    if (!s || !poses || n <= 0 || n > s->h.N) return QPSO_E_INVAL;
+    qpso_ctx* ctx = s->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int N = s->h.N;
+    char* A = s->arena;
+    for (int d = 0; d < 7; ++d)
+        HIP_TRY(ctx, hipMemcpyAsync(A + s->L.pos + sizeof(float) * (size_t)d * N, poses + (size_t)d * n,
+                                    sizeof(float) * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemsetAsync(A + s->L.iter, 0, sizeof(int32_t), st));
+    HIP_TRY(ctx, hipMemsetAsync(A + s->L.flags, 0, sizeof(int32_t), st));
+    HIP_TRY(ctx, hipMemsetAsync(A + s->L.cnt0, 0, sizeof(int32_t) * N, st));
+    HIP_TRY(ctx, hipMemsetAsync(A + s->L.counters, 0, sizeof(unsigned long long) * 3, st));
+    int32_t* d_frames = nullptr;
+    int32_t* d_aabb = nullptr;
+    QpsoSolveDev* d_tmp = nullptr;
+    auto cleanup = [&]() {
+        hipStreamSynchronize(st);
+        if (d_frames) hipFree(d_frames);
+        if (d_aabb) hipFree(d_aabb);
+        if (d_tmp) hipFree(d_tmp);
+    };
+    QpsoSolveDev tmp = s->h;
+    if (frames) {
+        if (hipMalloc(&d_frames, sizeof(int32_t) * (size_t)n * kFrame) != hipSuccess) { cleanup(); return QPSO_E_NOMEM; }
+        hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(d_frames), QPSO_EMPTY, (size_t)n * kFrame, st);
+        tmp.dbg_frames = d_frames;
+    }
+    if (hipMalloc(&d_aabb, sizeof(int32_t) * 4 * (size_t)n) != hipSuccess) { cleanup(); return QPSO_E_NOMEM; }
+    tmp.dbg_aabb = d_aabb;
+    if (hipMalloc(&d_tmp, sizeof(QpsoSolveDev)) != hipSuccess) { cleanup(); return QPSO_E_NOMEM; }
+    hipMemcpyAsync(d_tmp, &tmp, sizeof(tmp), hipMemcpyHostToDevice, st);
+    hipError_t e = qpso_launch::render_fitness(ctx->rf_cfg, d_tmp,
+                                               reinterpret_cast<const QpsoItem*>(A + s->L.items), n, st);
+    if (e == hipSuccess && fit) e = hipMemcpyAsync(fit, A + s->L.fit, sizeof(float) * n, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && aabb) e = hipMemcpyAsync(aabb, d_aabb, sizeof(int32_t) * 4 * n, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && frames)
+        e = hipMemcpyAsync(frames, d_frames, sizeof(int32_t) * (size_t)n * kFrame, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    cleanup();
+    if (e != hipSuccess) return set_err(ctx, QPSO_E_HIP, std::string("eval_poses: ") + hipGetErrorString(e));
+    return QPSO_OK;
+}
+
+// ---- batched form ---------------------------------------------------------
+int qpso_solve_run_batch(qpso_ctx* ctx, int n, qpso_solve* const* solves, qpso_result* out) {
+    if (!ctx || n <= 0 || !solves) return QPSO_E_INVAL;
+    for (int i = 0; i < n; ++i)
+        if (!solves[i] || solves[i]->ctx != ctx) return QPSO_E_INVAL;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    auto& P = ctx->plan;
+    const bool same = P.exec && (int)P.solves.size() == n && std::equal(P.solves.begin(), P.solves.end(), solves);
+    if (!same) {
+        invalidate_plan(ctx);
+        P.solves.assign(solves, solves + n);
+        std::vector<QpsoSolveDev> descs(n);
+        std::vector<int> order(n);
+        for (int i = 0; i < n; ++i) { descs[i] = solves[i]->h; order[i] = i; P.max_iters = std::max(P.max_iters, solves[i]->h.I); }
+        // heaviest solves first so the longest workgroups start early
+        auto cost = [&](int i) { return (double)descs[i].F_eff * descs[i].N; };
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+        std::vector<QpsoItem> items;
+        for (int i : order)
+            for (int q = 0; q < descs[i].N; ++q) items.push_back(QpsoItem{i, q});
+        P.n_items = (int)items.size();
+        HIP_TRY(ctx, hipMalloc(&P.d_descs, sizeof(QpsoSolveDev) * n));
+        HIP_TRY(ctx, hipMalloc(&P.d_items, sizeof(QpsoItem) * items.size()));
+        HIP_TRY(ctx, hipMemcpy(P.d_descs, descs.data(), sizeof(QpsoSolveDev) * n, hipMemcpyHostToDevice));
+        HIP_TRY(ctx, hipMemcpy(P.d_items, items.data(), sizeof(QpsoItem) * items.size(), hipMemcpyHostToDevice));
+        hipGraph_t graph = nullptr;
+        HIP_TRY(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipError_t e = qpso_launch::init(P.d_descs, n, st);
+        for (int it = 0; it < P.max_iters && e == hipSuccess; ++it) {
+            e = qpso_launch::render_fitness(ctx->rf_cfg, P.d_descs, P.d_items, P.n_items, st);
+            if (e == hipSuccess) e = qpso_launch::pso_step(P.d_descs, n, st);
+        }
+        hipError_t e2 = hipStreamEndCapture(st, &graph);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (graph) hipGraphDestroy(graph);
+            invalidate_plan(ctx);
+            return set_err(ctx, QPSO_E_HIP, "batch graph capture failed");
+        }
+        e = hipGraphInstantiate(&P.exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (e != hipSuccess) { invalidate_plan(ctx); return set_err(ctx, QPSO_E_HIP, "graph instantiate failed"); }
+    }
+    HIP_TRY(ctx, hipGraphLaunch(P.exec, st));
+    int rc = QPSO_OK;
+    for (int i = 0; i < n; ++i) {
+        int r = read_result(solves[i], out ? &out[i] : nullptr, st);
+        if (r && !rc) rc = r;
+    }
+    return rc;
+}
+
+}  // extern "C"
+
+// ---- serving: continuous batching under an HBM budget -----------------------
+struct qpso_server {
+    qpso_ctx* ctx = nullptr;
+    uint64_t budget = 0;
+    uint64_t used = 0;
+    hipStream_t st = nullptr;
+    std::thread th;
+    std::mutex mu;
+    std::condition_variable cv, done_cv;
+    bool stop = false;
+    struct Req {
+        uint64_t ticket;
+        uint32_t mesh_id;
+        qpso_params p;
+        qpso_cluster cl;
+        std::vector<uint16_t> depth;
+        uint64_t bytes;
+        std::chrono::steady_clock::time_point t_submit;
+    };
+    std::deque<Req> pending;
+    struct Active {
+        Req req;
+        qpso_solve* s;
+    };
+    std::vector<Active> active;
+    std::unordered_map<uint64_t, qpso_result> results;
+    std::vector<double> latencies_ms;
+    uint64_t next_ticket = 1, submitted = 0, completed = 0, rejected = 0, steps = 0;
+    double active_sum = 0;
+    std::chrono::steady_clock::time_point t_first, t_last;
+    bool have_first = false;
+};
+
+namespace {
+
+void server_loop(qpso_server* sv) {
+    qpso_ctx* ctx = sv->ctx;
+    hipSetDevice(ctx->device);
+    QpsoSolveDev* d_descs = nullptr;
+    QpsoItem* d_items = nullptr;
+    size_t cap_descs = 0, cap_items = 0;
+    std::vector<QpsoSolveDev> descs;
+    std::vector<QpsoItem> items;
+    for (;;) {
+        std::vector<qpso_server::Req> admit;
+        {
+            std::unique_lock<std::mutex> lk(sv->mu);
+            sv->cv.wait(lk, [&] { return sv->stop || !sv->pending.empty() || !sv->active.empty(); });
+            if (sv->stop && sv->active.empty()) break;
+            // admit in arrival order while the budget allows
+            while (!sv->pending.empty() && sv->used + sv->pending.front().bytes <= sv->budget) {
+                sv->used += sv->pending.front().bytes;
+                admit.push_back(std::move(sv->pending.front()));
+                sv->pending.pop_front();
+            }
+        }
+        std::vector<qpso_server::Active> fresh;
+        for (auto& r : admit) {
+            r.cl.depth = r.depth.data();
+            qpso_solve* s = nullptr;
+            const int rc = qpso_solve_create(ctx, r.mesh_id, &r.cl, &r.p, 0, &s);
+            if (rc) {
+                std::lock_guard<std::mutex> lk(sv->mu);
+                qpso_result res{};
+                res.status = rc;
+                sv->results[r.ticket] = res;
+                sv->used -= r.bytes;
+                ++sv->rejected;
+                sv->done_cv.notify_all();
+                continue;
+            }
+            fresh.push_back({std::move(r), s});
+        }
+        // new solves: InitQPSO on the server stream, then they join the step
+        for (auto& a : fresh) qpso_launch::init(a.s->d_desc, 1, sv->st);
+        for (auto& a : fresh) sv->active.push_back(std::move(a));
+        if (sv->active.empty()) continue;
+
+        // one device step = one PSO iteration of every resident solve
+        descs.clear();
+        items.clear();
+        for (size_t i = 0; i < sv->active.size(); ++i) {
+            descs.push_back(sv->active[i].s->h);
+            for (int q = 0; q < sv->active[i].s->h.N; ++q) items.push_back(QpsoItem{(int32_t)i, q});
+        }
+        if (descs.size() > cap_descs) {
+            if (d_descs) hipFree(d_descs);
+            cap_descs = descs.size() * 2;
+            hipMalloc(&d_descs, sizeof(QpsoSolveDev) * cap_descs);
+        }
+        if (items.size() > cap_items) {
+            if (d_items) hipFree(d_items);
+            cap_items = items.size() * 2;
+            hipMalloc(&d_items, sizeof(QpsoItem) * cap_items);
+        }
+        hipMemcpyAsync(d_descs, descs.data(), sizeof(QpsoSolveDev) * descs.size(), hipMemcpyHostToDevice, sv->st);
+        hipMemcpyAsync(d_items, items.data(), sizeof(QpsoItem) * items.size(), hipMemcpyHostToDevice, sv->st);
+        qpso_launch::render_fitness(ctx->rf_cfg, d_descs, d_items, (int)items.size(), sv->st);
+        qpso_launch::pso_step(d_descs, (int)descs.size(), sv->st);
+        std::vector<int32_t> iters(sv->active.size());
+        for (size_t i = 0; i < sv->active.size(); ++i)
+            hipMemcpyAsync(&iters[i], sv->active[i].s->h.iter, sizeof(int32_t), hipMemcpyDeviceToHost, sv->st);
+        hipStreamSynchronize(sv->st);
+        ++sv->steps;
+        sv->active_sum += (double)sv->active.size();
+
+        std::vector<qpso_server::Active> keep;
+        for (size_t i = 0; i < sv->active.size(); ++i) {
+            auto& a = sv->active[i];
+            if (iters[i] < a.s->h.I) { keep.push_back(std::move(a)); continue; }
+            qpso_result res{};
+            const int rc = read_result(a.s, &res, sv->st);
+            if (rc && res.status == QPSO_OK) res.status = rc;
+            const auto now = std::chrono::steady_clock::now();
+            qpso_solve_destroy(a.s);
+            std::lock_guard<std::mutex> lk(sv->mu);
+            sv->results[a.req.ticket] = res;
+            sv->latencies_ms.push_back(std::chrono::duration<double, std::milli>(now - a.req.t_submit).count());
+            sv->used -= a.req.bytes;
+            ++sv->completed;
+            sv->t_last = now;
+            sv->done_cv.notify_all();
+        }
+        sv->active.swap(keep);
+    }
+    if (d_descs) hipFree(d_descs);
+    if (d_items) hipFree(d_items);
+}
+
+}  // namespace
+
+extern "C" {
+
+int qpso_server_create(qpso_ctx* ctx, uint64_t hbm_budget_bytes, qpso_server** out) {
+    if (!ctx || !out || hbm_budget_bytes == 0) return QPSO_E_INVAL;
+    auto* sv = new qpso_server();
+    sv->ctx = ctx;
+    sv->budget = hbm_budget_bytes;
+    hipSetDevice(ctx->device);
+    if (hipStreamCreateWithFlags(&sv->st, hipStreamNonBlocking) != hipSuccess) {
+        delete sv;
+        return QPSO_E_HIP;
+    }
+    sv->th = std::thread(server_loop, sv);
+    *out = sv;
+    return QPSO_OK;
+}
+
+int qpso_server_submit(qpso_server* sv, uint32_t mesh_id, const qpso_cluster* cl, const qpso_params* p,
+                       uint64_t* ticket) {
+    if (!sv || !ticket) return QPSO_E_INVAL;
+    if (int v = validate(cl, p)) return v;
+    qpso_server::Req r;
+    r.mesh_id = mesh_id;
+    r.p = *p;
+    r.cl = *cl;
+    r.depth.assign(cl->depth, cl->depth + kFrame);
+    r.bytes = qpso_solve_bytes(p, cl->rect);
+    r.t_submit = std::chrono::steady_clock::now();
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (r.bytes > sv->budget) { ++sv->rejected; return QPSO_E_BUDGET; }
+    r.ticket = sv->next_ticket++;
+    *ticket = r.ticket;
+    if (!sv->have_first) { sv->t_first = r.t_submit; sv->have_first = true; }
+    ++sv->submitted;
+    sv->pending.push_back(std::move(r));
+    sv->cv.notify_all();
+    return QPSO_OK;
+}
+
+int qpso_server_poll(qpso_server* sv, uint64_t ticket, qpso_result* out) {
+    if (!sv) return QPSO_E_INVAL;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    auto it = sv->results.find(ticket);
+    if (it == sv->results.end()) return ticket < sv->next_ticket ? QPSO_E_PENDING : QPSO_E_NOTFOUND;
+    if (out) *out = it->second;
+    const int st = it->second.status;
+    sv->results.erase(it);
+    return st;
+}
+
+int qpso_server_wait(qpso_server* sv, uint64_t ticket, qpso_result* out) {
+    if (!sv) return QPSO_E_INVAL;
+    std::unique_lock<std::mutex> lk(sv->mu);
+    if (ticket == 0 || ticket >= sv->next_ticket) return QPSO_E_NOTFOUND;
+    sv->done_cv.wait(lk, [&] { return sv->results.count(ticket) != 0; });
+    auto it = sv->results.find(ticket);
+    if (out) *out = it->second;
+    const int st = it->second.status;
+    sv->results.erase(it);
+    return st;
+}
+
+int qpso_server_stats_get(qpso_server* sv, qpso_server_stats* st) {
+    if (!sv || !st) return QPSO_E_INVAL;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    std::memset(st, 0, sizeof(*st));
+    st->submitted = sv->submitted;
+    st->completed = sv->completed;
+    st->rejected = sv->rejected;
+    st->steps = sv->steps;
+    st->mean_active = sv->steps ? sv->active_sum / (double)sv->steps : 0.0;
+    if (!sv->latencies_ms.empty()) {
+        std::vector<double> v = sv->latencies_ms;
+        std::sort(v.begin(), v.end());
+        auto q = [&](double f) { return v[std::min(v.size() - 1, (size_t)(f * (double)(v.size() - 1) + 0.5))]; };
+        st->p50_ms = q(0.50);
+        st->p99_ms = q(0.99);
+        double s = 0;
+        for (double x : v) s += x;
+        st->mean_ms = s / (double)v.size();
+        const double span = std::chrono::duration<double>(sv->t_last - sv->t_first).count();
+        st->solves_per_s = span > 0 ? (double)sv->completed / span : 0.0;
+    }
+    return QPSO_OK;
+}
+
+int qpso_server_destroy(qpso_server* sv) {
+    if (!sv) return QPSO_E_INVAL;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stop = true;
+        sv->pending.clear();
+    }
+    sv->cv.notify_all();
+    if (sv->th.joinable()) sv->th.join();
+    hipStreamDestroy(sv->st);
+    delete sv;
+    return QPSO_OK;
+}
+
+}  // extern "C"

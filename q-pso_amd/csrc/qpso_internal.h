@@ -1,0 +1,66 @@
+// qpso_internal.h -- device-visible solve descriptor shared by the host code
+// (qpso_api.cpp) and the kernels (qpso_kernels.hip).  Plain C layout.
+#ifndef QPSO_INTERNAL_H_
+#define QPSO_INTERNAL_H_
+
+#include <stdint.h>
+
+#define QPSO_EMPTY 60000          // maxVal_depthBufferInt (main.h:148)
+#define QPSO_AABB_MIN_SENT 50000  // max_AABB_ (main.h:150)
+#define QPSO_AABB_MAX_SENT (-1)   // min_AABB_ (main.h:151)
+#define QPSO_BEST_FIT_INIT 1.0e10f
+#define QPSO_FXP_LIMIT (1 << 30)  // degenerate guard on 28.4 coordinates
+#define QPSO_AREA_LIMIT (1 << 23) // degenerate guard on the union box area
+
+// flags[] bits
+#define QPSO_FLAG_STALE_OVERFLOW 1
+
+struct QpsoStale {  // one carried-over depth-buffer pixel
+    int32_t pos;    // (y << 16) | x
+    int32_t val;
+};
+
+struct QpsoItem {  // one render+fitness workgroup: (solve index, particle index)
+    int32_t solve, particle;
+};
+
+struct QpsoSolveDev {
+    // mesh (metres, loadOBJ3 SoA)
+    const float* mesh;
+    int32_t F;       // faces in the SoA (row stride)
+    int32_t F_eff;   // faces rendered (face cap)
+    // camera
+    double fx, fy, cx, cy;
+    // PSO
+    int32_t N, I, S;  // particles, iterations, random stride
+    double w;
+    float c1, c2;
+    float x_lo[3], x_hi[3];
+    // cluster
+    int32_t rect[4];        // minx, miny, maxx, maxy (max exclusive)
+    uint32_t num_points;
+    const uint16_t* obs;    // rect crop of the cluster depth, pitch rect[2]-rect[0]
+    // state
+    const float* rand;      // [S][N]
+    float* pos;             // [7][N]
+    float* vel;
+    float* pbest;
+    float* pbest_fit;       // [N]
+    float* fit;             // [N]
+    float* gbest;           // [8]: pose + fit
+    int32_t* iter;          // [1]
+    QpsoStale* stale[2];    // [N][stale_cap] each
+    int32_t* stale_cnt[2];  // [N] each
+    int32_t stale_cap;
+    int32_t* flags;         // [1]
+    unsigned long long* counters;  // [3]: raster px, visit px, degenerate evals
+    // traces (nullable)
+    float* fit_trace;       // [I][N]
+    float* gbest_trace;     // [I][8]
+    int32_t* argmin_trace;  // [I]
+    // debug outputs of the stage entry point (nullable)
+    int32_t* dbg_frames;    // [N][480][640]
+    int32_t* dbg_aabb;      // [N][4]
+};
+
+#endif

@@ -1,0 +1,220 @@
+// qpso_io.cpp -- mesh/depth readers, result writers, CPU visualisation render.
+#include "qpso_io.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace {
+constexpr int kRows = QPSO_ROWS;
+constexpr int kCols = QPSO_COLS;
+}
+
+extern "C" {
+
+void qpso_free(void* p) { std::free(p); }
+
+int qpso_read_obj(const char* path, float** soa_out, uint32_t* n_faces) {
+    if (!path || !soa_out || !n_faces) return QPSO_E_INVAL;
+    std::ifstream in(path);
+    if (!in) return QPSO_E_NOTFOUND;
+    std::vector<float> v;                // all vertices (global indexing)
+    std::vector<int> tri;                // 3 vertex indices per face
+    std::string line;
+    int objects = 0;
+    while (std::getline(in, line)) {
+        std::istringstream ss(line);
+        std::string tag;
+        if (!(ss >> tag)) continue;
+        if (tag == "o") {
+            if (++objects > 1 && !tri.empty()) break;  // first object only (loadOBJ3: mMeshes[0])
+        } else if (tag == "v") {
+            float x = 0, y = 0, z = 0;
+            ss >> x >> y >> z;
+            v.push_back(x); v.push_back(y); v.push_back(z);
+        } else if (tag == "f") {
+            std::vector<int> poly;
+            std::string tok;
+            while (ss >> tok) {
+                const int idx = std::atoi(tok.c_str());  // "a", "a/b", "a//c", "a/b/c"
+                if (idx == 0) return QPSO_E_INVAL;
+                poly.push_back(idx > 0 ? idx - 1 : (int)(v.size() / 3) + idx);
+            }
+            for (size_t k = 2; k < poly.size(); ++k) {  // fan triangulation
+                tri.push_back(poly[0]); tri.push_back(poly[k - 1]); tri.push_back(poly[k]);
+            }
+        }
+    }
+    const size_t F = tri.size() / 3;
+    if (F == 0) return QPSO_E_INVAL;
+    float* A = (float*)std::malloc(sizeof(float) * 9 * F);
+    if (!A) return QPSO_E_NOMEM;
+    for (size_t f = 0; f < F; ++f)
+        for (int j = 0; j < 3; ++j) {
+            const int i = tri[f * 3 + j];
+            if (i < 0 || (size_t)i * 3 + 2 >= v.size()) { std::free(A); return QPSO_E_INVAL; }
+            A[f + j * F] = v[i * 3 + 0];
+            A[3 * F + f + j * F] = v[i * 3 + 1];
+            A[6 * F + f + j * F] = v[i * 3 + 2];
+        }
+    *soa_out = A;
+    *n_faces = (uint32_t)F;
+    return QPSO_OK;
+}
+
+int qpso_read_dpt(const char* path, uint16_t** data_out, int32_t* rows, int32_t* cols) {
+    if (!path || !data_out || !rows || !cols) return QPSO_E_INVAL;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return QPSO_E_NOTFOUND;
+    int32_t r = 0, c = 0;
+    if (std::fread(&r, 4, 1, f) != 1 || std::fread(&c, 4, 1, f) != 1 || r <= 0 || c <= 0 || r > 16384 ||
+        c > 16384) {
+        std::fclose(f);
+        return QPSO_E_INVAL;
+    }
+    uint16_t* d = (uint16_t*)std::malloc(sizeof(uint16_t) * (size_t)r * c);
+    if (!d) { std::fclose(f); return QPSO_E_NOMEM; }
+    const size_t n = std::fread(d, sizeof(uint16_t), (size_t)r * c, f);
+    std::fclose(f);
+    if (n != (size_t)r * c) { std::free(d); return QPSO_E_INVAL; }
+    *data_out = d;
+    *rows = r;
+    *cols = c;
+    return QPSO_OK;
+}
+
+int qpso_write_dpt(const char* path, const uint16_t* data, int32_t rows, int32_t cols) {
+    if (!path || !data || rows <= 0 || cols <= 0) return QPSO_E_INVAL;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return QPSO_E_NOTFOUND;
+    bool ok = std::fwrite(&rows, 4, 1, f) == 1 && std::fwrite(&cols, 4, 1, f) == 1 &&
+              std::fwrite(data, sizeof(uint16_t), (size_t)rows * cols, f) == (size_t)rows * cols;
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? QPSO_OK : QPSO_E_INVAL;
+}
+
+int qpso_write_bestpose(const char* path, float fit, const float pose[7]) {
+    FILE* out = std::fopen(path, "w");
+    if (!out) return QPSO_E_NOTFOUND;
+    std::fprintf(out, "%f,%f,%f,%f,%f,%f,%f,%f", fit, pose[0], pose[1], pose[2], pose[3], pose[4], pose[5],
+                 pose[6]);
+    std::fclose(out);
+    return QPSO_OK;
+}
+
+// RenderCPU / CPUfixedPointRender (main.h:1810-1991): host visualisation,
+// float depth buffer, iround = (int)(v + 0.5f), inside test dead as on the GPU.
+int qpso_render_cpu(const float pose[7], const float* A, uint32_t n_faces, const qpso_camera* cam,
+                    float* buf) {
+    if (!pose || !A || !cam || !buf) return QPSO_E_INVAL;
+    const size_t F = n_faces;
+    auto iround = [](float v) { return (int)(v + 0.5f); };
+    for (size_t i = 0; i < F; ++i) {
+        float U[3], V[3], Z[3];
+        for (int j = 0; j < 3; ++j) {
+            const float vx = A[i + j * F], vy = A[3 * F + i + j * F], vz = A[6 * F + i + j * F];
+            const float q0 = pose[3], q1 = pose[4], q2 = pose[5], q3 = pose[6];
+            const float t0 = 2.0f * (q2 * vz - q3 * vy);
+            const float t1 = 2.0f * -(q1 * vz - q3 * vx);
+            const float t2 = 2.0f * (q1 * vy - q2 * vx);
+            const float px = vx + q0 * t0 + (q2 * t2 - q3 * t1) + pose[0];
+            const float py = vy + q0 * t1 + -(q1 * t2 - q3 * t0) + pose[1];
+            const float pz = vz + q0 * t2 + (q1 * t1 - q2 * t0) + pose[2];
+            const float r = 1.f / pz;
+            U[j] = (float)(cam->cx + cam->fx * (px * r));
+            V[j] = (float)(cam->cy + cam->fy * (py * r));
+            Z[j] = 1000.0f * pz;
+        }
+        const int X1 = iround(16.f * U[0]), X2 = iround(16.f * U[1]), X3 = iround(16.f * U[2]);
+        const int Y1 = iround(16.f * V[0]), Y2 = iround(16.f * V[1]), Y3 = iround(16.f * V[2]);
+        const int DX12 = X1 - X2, DX23 = X2 - X3, DX31 = X3 - X1;
+        const int DY12 = Y1 - Y2, DY23 = Y2 - Y3, DY31 = Y3 - Y1;
+        const int tri = DX12 * DY31 - DX31 * DY12;
+        if (tri <= 0) continue;
+        const float inv = 1.0f / float(tri);
+        const int minx = (std::min(X1, std::min(X2, X3)) + 0xF) >> 4, maxx = (std::max(X1, std::max(X2, X3)) + 0xF) >> 4;
+        const int miny = (std::min(Y1, std::min(Y2, Y3)) + 0xF) >> 4, maxy = (std::max(Y1, std::max(Y2, Y3)) + 0xF) >> 4;
+        if (maxx - minx <= 0 || maxy - miny <= 0) continue;
+        int C2 = DY23 * X2 - DX23 * Y2 + (DY23 < 0 || (DY23 == 0 && DX23 > 0));
+        int C3 = DY31 * X3 - DX31 * Y3 + (DY31 < 0 || (DY31 == 0 && DX31 > 0));
+        int CY2 = C2 + DX23 * (miny << 4) - DY23 * (minx << 4);
+        int CY3 = C3 + DX31 * (miny << 4) - DY31 * (minx << 4);
+        const float z2 = (Z[1] - Z[0]) * inv, z3 = (Z[2] - Z[0]) * inv;
+        for (int y = miny; y < maxy; ++y) {
+            int CX2 = CY2, CX3 = CY3;
+            for (int x = minx; x < maxx; ++x) {
+                const int cx2 = (CX2 + 0xF) >> 8, cx3 = (CX3 + 0xF) >> 8;
+                float d = Z[0];
+                d += cx2 * z2;
+                d += cx3 * z3;
+                if (x >= 0 && x < kCols && y >= 0 && y < kRows) {  // frame clip
+                    float& b = buf[y * kCols + x];
+                    if (d < b) b = d;
+                }
+                CX2 -= DY23 << 4;
+                CX3 -= DY31 << 4;
+            }
+            CY2 += DX23 << 4;
+            CY3 += DX31 << 4;
+        }
+    }
+    return QPSO_OK;
+}
+
+int qpso_render_cpu_csv(const char* path, const float pose[7], const float* soa, uint32_t n_faces,
+                        const qpso_camera* cam) {
+    std::vector<float> buf(kRows * kCols, 60000.0f);  // maxVal_depthBuffer
+    int rc = qpso_render_cpu(pose, soa, n_faces, cam, buf.data());
+    if (rc) return rc;
+    std::ofstream ofs(path);
+    if (!ofs) return QPSO_E_NOTFOUND;
+    for (int r = 0; r < kRows; ++r) {
+        for (int c = 0; c < kCols; ++c) {
+            ofs << buf[r * kCols + c];
+            if (c < kCols - 1) ofs << ", ";
+        }
+        ofs << "\n";
+    }
+    return QPSO_OK;
+}
+
+int qpso_clusters_from_labels(const uint16_t* labels, const uint16_t* depth, int k, const qpso_camera* cam,
+                              int32_t rect[4], float centroid_mm[3], uint32_t* num_points,
+                              uint16_t* depth_out) {
+    if (!labels || !depth || !cam || !rect || !centroid_mm || !num_points || k < 0) return QPSO_E_INVAL;
+    const uint16_t want = (uint16_t)(k + 1);
+    int minx = kCols, miny = kRows, maxx = -1, maxy = -1;
+    uint32_t n = 0, nz = 0;
+    double sx = 0, sy = 0, sz = 0;
+    for (int y = 0; y < kRows; ++y)
+        for (int x = 0; x < kCols; ++x) {
+            const int i = y * kCols + x;
+            const bool in = labels[i] == want;
+            if (depth_out) depth_out[i] = in ? depth[i] : 0;
+            if (!in) continue;
+            ++n;
+            minx = std::min(minx, x); maxx = std::max(maxx, x);
+            miny = std::min(miny, y); maxy = std::max(maxy, y);
+            if (depth[i] == 0) continue;
+            const double z = depth[i];
+            sx += (x - cam->cx) * z / cam->fx;
+            sy += (y - cam->cy) * z / cam->fy;
+            sz += z;
+            ++nz;
+        }
+    if (n == 0 || nz == 0) return QPSO_E_NOTFOUND;
+    rect[0] = minx; rect[1] = miny; rect[2] = maxx + 1; rect[3] = maxy + 1;
+    centroid_mm[0] = (float)(sx / nz);
+    centroid_mm[1] = (float)(sy / nz);
+    centroid_mm[2] = (float)(sz / nz);
+    *num_points = n;
+    return QPSO_OK;
+}
+
+}  // extern "C"

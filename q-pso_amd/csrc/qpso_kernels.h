@@ -1,0 +1,26 @@
+// qpso_kernels.h -- host-side launchers of the gfx950 kernels.
+#ifndef QPSO_KERNELS_H_
+#define QPSO_KERNELS_H_
+
+#include <hip/hip_runtime_api.h>
+
+#include "qpso_internal.h"
+
+namespace qpso_launch {
+
+// Render+fitness kernel variants: (threads per workgroup, int32 tile capacity).
+int rf_num_configs();
+int rf_config_threads(int cfg);
+int rf_config_tile(int cfg);
+
+// InitQPSO for every solve in d_solves[0..n).
+hipError_t init(const QpsoSolveDev* d_solves, int n_solves, hipStream_t st);
+// One fused render+fitness pass; d_items[i] = (solve index, particle index).
+hipError_t render_fitness(int cfg, const QpsoSolveDev* d_solves, const QpsoItem* d_items, int n_items,
+                          hipStream_t st);
+// argmin + pbest/gbest + particle update, one workgroup per solve.
+hipError_t pso_step(const QpsoSolveDev* d_solves, int n_solves, hipStream_t st);
+
+}  // namespace qpso_launch
+
+#endif

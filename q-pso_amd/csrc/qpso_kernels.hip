@@ -1,0 +1,543 @@
+// qpso_kernels.hip -- gfx950 kernels of the Q-PSO pose-search hot path.
+//
+//   k_init            InitQPSO (main.cu:72-198) for every particle of every
+//                     listed solve; resets the solve's gbest, iteration
+//                     counter and stale-pixel lists.
+//   k_render_fitness  one workgroup per (solve, particle): fuses
+//                     RenderFixedPointKernelAABB (main.cu:305-419),
+//                     ComputeAABBFinalReductionKernel (main.cu:423-546) and
+//                     ComputeFitnessKernel (main.cu:550-764).  The depth
+//                     buffer is an int32 tile in LDS covering the in-frame
+//                     part of the union box, processed in row bands when the
+//                     box exceeds the tile; ds_min_i32 replaces the
+//                     reference's global atomicMin.
+//   k_pso_step        one workgroup per solve: thrust::min_element argmin
+//                     (main.cu:1691-1698), UpdatePersonalAndGlobalBest
+//                     (main.cu:768-855) and UpdateParticlesEquationKernel
+//                     (main.cu:858-1060), with no host round trip.
+#include <hip/hip_runtime.h>
+
+#include "qpso_internal.h"
+#include "qpso_kernels.h"
+#include "qpso_math.h"
+
+using namespace qpso;
+
+namespace {
+
+constexpr int kRows = 480;
+constexpr int kCols = 640;
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ unsigned wave_sum(unsigned v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += (unsigned)__shfl_xor((int)v, o);
+    return v;
+}
+
+#define RT(S, p, s) (S).rand[(size_t)(p) + (size_t)(s) * (size_t)(S).N]
+
+// ------------------------------------------------------------------------
+// InitQPSO (main.cu:72-198) + QuatKinematicsPSOStateVectWithInit (main.h:267-307)
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_init(const QpsoSolveDev* __restrict__ solves) {
+    const QpsoSolveDev& S = solves[blockIdx.x];
+    const int N = S.N;
+    for (int p = threadIdx.x; p < N; p += blockDim.x) {
+        float P[7], V[7];
+        int s = 0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            P[d] = __builtin_fmaf(S.x_hi[d] - S.x_lo[d], RT(S, p, s), S.x_lo[d]); ++s;
+            V[d] = __builtin_fmaf(6.0f, RT(S, p, s), -3.0f); ++s;  // -vdot_ + 2 vdot_ r
+        }
+        const float qinit[4] = {0.7071f, 0.7071f, 0.0f, 0.0f};
+        float wp[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { wp[d] = __builtin_fmaf(20.0f, RT(S, p, s), -10.0f); ++s; }
+        float q[4];
+        kinematics_from(qinit, q, wp);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) P[3 + i] = q[i];
+        V[3] = 0.0f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { V[4 + d] = __builtin_fmaf(20.0f, RT(S, p, s), -10.0f); ++s; }
+#pragma unroll
+        for (int d = 0; d < 7; ++d) {
+            S.pos[d * N + p] = P[d];
+            S.vel[d * N + p] = V[d];
+            S.pbest[d * N + p] = P[d];
+        }
+        S.pbest_fit[p] = QPSO_BEST_FIT_INIT;
+        S.stale_cnt[0][p] = 0;
+        S.stale_cnt[1][p] = 0;
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int d = 0; d < 7; ++d) S.gbest[d] = 0.0f;
+        S.gbest[7] = QPSO_BEST_FIT_INIT;
+        *S.iter = 0;
+        *S.flags = 0;
+        S.counters[0] = 0;
+        S.counters[1] = 0;
+        S.counters[2] = 0;
+    }
+}
+
+// ------------------------------------------------------------------------
+// fused render + AABB + fitness
+// ------------------------------------------------------------------------
+template <int BT, int CAP>
+__global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __restrict__ solves,
+                                                      const QpsoItem* __restrict__ items) {
+    constexpr int SPT = 2048 / BT;  // fitness slots per thread
+    __shared__ int32_t tile[CAP];
+    __shared__ int32_t s_box[4];
+    __shared__ int32_t s_degen;
+    __shared__ int32_t s_nstale;
+    __shared__ uint32_t s_cnt[3];  // seg, tr, n_rendered
+    __shared__ unsigned long long s_px[2];
+
+    const int tid = threadIdx.x;
+    const QpsoItem item = items[blockIdx.x];
+    const QpsoSolveDev& S = solves[item.solve];
+    const int p = item.particle;
+    const int iter = *S.iter;
+    if (iter >= S.I) return;  // this solve has finished (batched runs)
+
+    const int N = S.N;
+    float pose[7];
+#pragma unroll
+    for (int d = 0; d < 7; ++d) pose[d] = S.pos[d * N + p];
+    const Cam cam{S.fx, S.fy, S.cx, S.cy};
+    const float* __restrict__ mesh = S.mesh;
+    const int F = S.F, Fe = S.F_eff;
+
+    if (tid == 0) {
+        s_box[0] = QPSO_AABB_MIN_SENT; s_box[1] = QPSO_AABB_MAX_SENT;
+        s_box[2] = QPSO_AABB_MIN_SENT; s_box[3] = QPSO_AABB_MAX_SENT;
+        s_degen = 0; s_nstale = 0;
+        s_cnt[0] = s_cnt[1] = s_cnt[2] = 0;
+        s_px[0] = s_px[1] = 0;
+    }
+    __syncthreads();
+
+    // ---- pass A: per-face bbox and AABB over every face (main.h:1340-1353,
+    //      AABBreduction main.h:1190-1304, final reduction main.cu:423-546)
+    {
+        int bx0 = QPSO_AABB_MIN_SENT, bx1 = QPSO_AABB_MAX_SENT;
+        int by0 = QPSO_AABB_MIN_SENT, by1 = QPSO_AABB_MAX_SENT;
+        int dg = 0;
+        for (int f = tid; f < Fe; f += BT) {
+            Face fc;
+            face_project(mesh, F, f, pose, cam, fc);
+            bx0 = min(bx0, fc.minx); bx1 = max(bx1, fc.maxx);
+            by0 = min(by0, fc.miny); by1 = max(by1, fc.maxy);
+            dg |= fc.degen ? 1 : 0;
+        }
+        bx0 = wave_min(bx0); bx1 = wave_max(bx1);
+        by0 = wave_min(by0); by1 = wave_max(by1);
+        dg = wave_max(dg);
+        if ((tid & 63) == 0) {
+            atomicMin(&s_box[0], bx0); atomicMax(&s_box[1], bx1);
+            atomicMin(&s_box[2], by0); atomicMax(&s_box[3], by1);
+            if (dg) s_degen = 1;
+        }
+    }
+    __syncthreads();
+
+    const int32_t minX = min(s_box[0], S.rect[0]), maxX = max(s_box[1], S.rect[2]);
+    const int32_t minY = min(s_box[2], S.rect[1]), maxY = max(s_box[3], S.rect[3]);
+    const long long W64 = (long long)maxX - minX, H64 = (long long)maxY - minY;
+    const bool degen = s_degen != 0 || W64 <= 0 || H64 <= 0 || W64 * H64 > (long long)QPSO_AREA_LIMIT;
+
+    if (S.dbg_aabb && tid == 0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) S.dbg_aabb[p * 4 + i] = s_box[i];
+    }
+    const int par = iter & 1;
+    const int cap = S.stale_cap;
+    const QpsoStale* __restrict__ old = S.stale[par] + (size_t)p * cap;
+    const int n_old = S.stale_cnt[par][p];
+    QpsoStale* __restrict__ nw = S.stale[par ^ 1] + (size_t)p * cap;
+
+    if (degen) {  // guard G: NaN fitness, buffer reset
+        if (tid == 0) {
+            S.fit[p] = __builtin_nanf("");
+            S.stale_cnt[par ^ 1][p] = 0;
+            atomicAdd(&S.counters[2], 1ull);
+        }
+        return;
+    }
+
+    const int32_t W = (int32_t)W64, H = (int32_t)H64;
+    const int32_t A = W * H;
+    const int32_t T = ((A + 1023) >> 10) << 10;  // NumIters * 1024 (main.cu:621-623)
+    const float rW = rcp((float)W);
+    // in-frame part of the union box: columns [tx0, tx1), rows [ty0, ty1]
+    const int32_t tx0 = max(minX, 0), tx1 = min(maxX, kCols);
+    const int32_t ty0 = max(minY, 0), ty1 = min(maxY, kRows - 1);
+    const int32_t TW = tx1 - tx0;
+    const int32_t BH = CAP / TW;
+    const int32_t rx0 = S.rect[0], ry0 = S.rect[1], rx1 = S.rect[2], ry1 = S.rect[3];
+    const int32_t rw = rx1 - rx0;
+    const uint16_t* __restrict__ obs = S.obs;
+
+    float acc_e[SPT];
+    uint32_t acc_s[SPT], acc_t[SPT];
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) { acc_e[j] = 0.0f; acc_s[j] = 0; acc_t[j] = 0; }
+    uint32_t n_rend = 0;
+    unsigned long long px_r = 0, px_f = 0;
+
+    auto push_stale = [&](int32_t x, int32_t y, int32_t v) {
+        const int k = atomicAdd(&s_nstale, 1);
+        if (k < cap) nw[k] = QpsoStale{(y << 16) | x, v};
+        else atomicOr(S.flags, QPSO_FLAG_STALE_OVERFLOW);
+    };
+    // y(kW) == k - 1: the fp32 row index of t = kW falls one row short (main.cu:645)
+    auto short_row = [&](int32_t k) -> bool { return k > 0 && row_of(k * W, rW) == k - 1; };
+
+    for (int32_t b0 = ty0; b0 <= ty1; b0 += BH) {
+        const int32_t b1 = min(b0 + BH, ty1 + 1);  // band rows [b0, b1)
+        const int32_t nt = (b1 - b0) * TW;
+        for (int i = tid; i < nt; i += BT) tile[i] = QPSO_EMPTY;
+        __syncthreads();
+        // pixels carried over from earlier iterations
+        for (int k = tid; k < n_old; k += BT) {
+            const QpsoStale e = old[k];
+            const int32_t ex = e.pos & 0xffff, ey = e.pos >> 16;
+            if (ey >= b0 && ey < b1 && ex >= tx0 && ex < tx1) tile[(ey - b0) * TW + ex - tx0] = e.val;
+        }
+        __syncthreads();
+
+        // ---- raster: bbox fill with ds_min_i32 (main.h:1355-1488)
+        for (int f = tid; f < Fe; f += BT) {
+            Face fc;
+            face_project(mesh, F, f, pose, cam, fc);
+            if (fc.maxy <= b0 || fc.miny >= b1) continue;
+            Setup su;
+            if (!face_setup(fc, su)) continue;
+            const int32_t x0 = max(fc.minx, tx0), x1 = min(fc.maxx, tx1);
+            const int32_t y0 = max(fc.miny, b0), y1 = min(fc.maxy, b1);
+            if (x0 >= x1 || y0 >= y1) continue;
+            px_r += (unsigned long long)((x1 - x0) * (y1 - y0));
+            for (int32_t y = y0; y < y1; ++y) {
+                const int32_t cy2 = wadd(su.CY2, wmul(y - fc.miny, su.FDX23));
+                const int32_t cy3 = wadd(su.CY3, wmul(y - fc.miny, su.FDX31));
+                int32_t CX2 = wsub(cy2, wmul(x0 - fc.minx, su.FDY23));
+                int32_t CX3 = wsub(cy3, wmul(x0 - fc.minx, su.FDY31));
+                int32_t* row = tile + (y - b0) * TW - tx0;
+                for (int32_t x = x0; x < x1; ++x) {
+                    atomicMin(&row[x], pixel_depth(CX2, CX3, su));
+                    CX2 = wsub(CX2, su.FDY23);
+                    CX3 = wsub(CX3, su.FDY31);
+                }
+            }
+        }
+        __syncthreads();
+
+        if (S.dbg_frames) {
+            int32_t* fr = S.dbg_frames + (size_t)p * kRows * kCols;
+            for (int i = tid; i < nt; i += BT) fr[(b0 + i / TW) * kCols + tx0 + i % TW] = tile[i];
+        }
+
+        // ---- fitness over the t whose row falls in this band (main.cu:632-733)
+        const int32_t k0 = b0 - minY, k1 = b1 - minY;
+        const int32_t tlo = min(k0 * W + (short_row(k0) ? 1 : 0), T);
+        const int32_t thi = min(k1 * W + (short_row(k1) ? 1 : 0), T);
+        for (int32_t base = (tlo / 2048) * 2048; base < thi; base += 2048) {
+#pragma unroll
+            for (int j = 0; j < SPT; ++j) {
+                const int32_t t = base + j * BT + tid;  // slot t & 2047 = j*BT + tid
+                if (t < tlo || t >= thi) continue;
+                const int32_t y = row_of(t, rW);
+                int32_t x = t - y * W;
+                if (x >= W) x -= W;  // t = kW mapped to row k-1
+                const int32_t px = minX + x, py = minY + y;
+                if (px < tx0 || px >= tx1) continue;  // off-frame column: empty
+                const int32_t zr = tile[(py - b0) * TW + px - tx0];
+                const int32_t zk = (px >= rx0 && px < rx1 && py >= ry0 && py < ry1)
+                                       ? (int32_t)obs[(py - ry0) * rw + (px - rx0)] : 0;
+                ++px_f;
+                if (zk != 0 && zr == QPSO_EMPTY) acc_s[j] += 1;
+                if (zr != QPSO_EMPTY) {
+                    ++n_rend;
+                    if (zk == 0) {
+                        acc_t[j] += 1;
+                    } else {
+                        const float e = (float)wsub(zr, zk) * 0.001f;
+                        acc_e[j] = __builtin_fmaf(e, e, acc_e[j]);
+                    }
+                }
+            }
+        }
+
+        // ---- pixels of this band the scan never visits keep their depth
+        if (minX >= tx0) {  // x-offset 0 column is in frame
+            for (int32_t py = b0 + tid; py < b1; py += BT) {
+                const int32_t k = py - minY;
+                const bool vis = (k * W < T && !short_row(k)) || ((k + 1) * W < T && short_row(k + 1));
+                if (!vis) {
+                    const int32_t v = tile[(py - b0) * TW + minX - tx0];
+                    if (v != QPSO_EMPTY) push_stale(minX, py, v);
+                }
+            }
+        }
+        if (maxY >= b0 && maxY < b1) {  // last row, tail beyond t = T
+            const int32_t xs = max(max(1, T - H * W), tx0 - minX), xe = min(W, tx1 - minX);
+            for (int32_t xb = xs + tid; xb < xe; xb += BT) {
+                const int32_t v = tile[(maxY - b0) * TW + minX + xb - tx0];
+                if (v != QPSO_EMPTY) push_stale(minX + xb, maxY, v);
+            }
+        }
+        __syncthreads();
+    }
+
+    // carried pixels outside this iteration's box stay as they are
+    for (int k = tid; k < n_old; k += BT) {
+        const QpsoStale e = old[k];
+        const int32_t ex = e.pos & 0xffff, ey = e.pos >> 16;
+        if (!(ey >= ty0 && ey <= ty1 && ex >= tx0 && ex < tx1)) push_stale(ex, ey, e.val);
+    }
+
+    // ---- reductions (FitnessReductionBallot, main.h:1522-1573): halving tree
+    //      over the 2048 error slots; thread tid holds slots tid + j*BT
+#pragma unroll
+    for (int stride = 1024; stride >= BT; stride >>= 1) {
+#pragma unroll
+        for (int j = 0; j < stride / BT; ++j) acc_e[j] = acc_e[j] + acc_e[j + stride / BT];
+    }
+    uint32_t cs = 0, ct = 0;
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) { cs += acc_s[j]; ct += acc_t[j]; }
+    cs = wave_sum(cs); ct = wave_sum(ct);
+    const uint32_t cr = wave_sum(n_rend);
+    if ((tid & 63) == 0) {
+        atomicAdd(&s_cnt[0], cs); atomicAdd(&s_cnt[1], ct); atomicAdd(&s_cnt[2], cr);
+    }
+    atomicAdd(&s_px[0], px_r);
+    atomicAdd(&s_px[1], px_f);
+
+    float* red = reinterpret_cast<float*>(tile);  // the tile is dead now
+    red[tid] = acc_e[0];
+#pragma unroll
+    for (int stride = BT / 2; stride >= 64; stride >>= 1) {
+        __syncthreads();
+        if (tid < stride) red[tid] = red[tid] + red[tid + stride];
+    }
+    __syncthreads();
+    if (tid < 64) {
+        float v = red[tid];
+#pragma unroll
+        for (int stride = 32; stride >= 1; stride >>= 1) {
+            const float o = __shfl_down(v, stride);
+            v = v + o;
+        }
+        if (tid == 0) {
+            const uint32_t seg = s_cnt[0], tr = s_cnt[1], nr = s_cnt[2];
+            const float inv_r = rcp((float)nr);
+            const float ratio = (float)seg * rcp((float)S.num_points);
+            const float ratioT = (float)tr * inv_r;
+            const float zerr = v * inv_r;
+            S.fit[p] = (float)__builtin_fma((double)(ratio + ratioT), 0.05, (double)zerr);
+            const int ns = s_nstale;
+            S.stale_cnt[par ^ 1][p] = ns < cap ? ns : cap;
+            atomicAdd(&S.counters[0], s_px[0]);
+            atomicAdd(&S.counters[1], s_px[1]);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// argmin + pbest/gbest + particle update, one workgroup per solve
+// ------------------------------------------------------------------------
+__device__ __forceinline__ bool better(float av, int ai, float bv, int bi) {
+    // invalid (NaN) entries carry index INT32_MAX
+    if (ai == INT32_MAX) return false;
+    if (bi == INT32_MAX) return true;
+    return av < bv || (av == bv && ai < bi);
+}
+
+__device__ void update_particle(const QpsoSolveDev& S, int p, int iter, const float* gb) {
+    const int N = S.N;
+    float P[7], V[7], B[7];
+#pragma unroll
+    for (int d = 0; d < 7; ++d) {
+        P[d] = S.pos[d * N + p];
+        V[d] = S.vel[d * N + p];
+        B[d] = S.pbest[d * N + p];
+    }
+    int s = 12 + 12 * iter;  // d_randIdx: 12 at init + 12 per iteration
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const float r1 = S.c1 * RT(S, p, s); ++s;
+        const float r2 = S.c2 * RT(S, p, s); ++s;
+        const float iV = (float)(S.w * (double)V[d]);
+        float v = __builtin_fmaf(r1, B[d] - P[d], iV);
+        v = __builtin_fmaf(r2, gb[d] - P[d], v);
+        V[d] = v;
+        P[d] += v;
+        if (P[d] < S.x_lo[d]) { P[d] = S.x_lo[d]; V[d] = 0.0f; }
+        else if (P[d] > S.x_hi[d]) { P[d] = S.x_hi[d]; V[d] = 0.0f; }
+    }
+    float wc[3] = {0.0f, 0.0f, 0.0f}, ws[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        float qt[4];
+        qprod_conj(k == 0 ? &B[3] : &gb[3], &P[3], qt);
+        top_half(qt);
+        float n2 = __builtin_fmaf(qt[1], qt[1], qt[2] * qt[2]);
+        n2 = __builtin_fmaf(qt[3], qt[3], n2);
+        const float n = __builtin_sqrtf(n2);
+        if (n > 1.0e-7f) {
+            const float sc = (2.0f * acosf_q(qt[0])) * rcp(n);
+            float* o = k == 0 ? wc : ws;
+            o[0] = qt[1] * sc; o[1] = qt[2] * sc; o[2] = qt[3] * sc;
+        }
+    }
+    V[3] = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float r1 = S.c1 * RT(S, p, s); ++s;
+        const float r2 = S.c2 * RT(S, p, s); ++s;
+        const float iW = (float)(S.w * (double)V[4 + i]);
+        const float v = __builtin_fmaf(r1, wc[i], iW);
+        V[4 + i] = __builtin_fmaf(r2, ws[i], v);
+    }
+    float q[4] = {P[3], P[4], P[5], P[6]};
+    kinematics_from(q, &P[3], &V[4]);
+#pragma unroll
+    for (int d = 0; d < 7; ++d) {
+        S.pos[d * N + p] = P[d];
+        S.vel[d * N + p] = V[d];
+    }
+}
+
+template <int BT>
+__global__ __launch_bounds__(BT) void k_pso_step(const QpsoSolveDev* __restrict__ solves) {
+    __shared__ float s_v[BT / 64];
+    __shared__ int s_i[BT / 64];
+    __shared__ float s_g[8];
+    __shared__ int s_best;
+    const QpsoSolveDev& S = solves[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int iter = *S.iter;
+    if (iter >= S.I) return;
+    const int N = S.N;
+
+    float bv = 0.0f;
+    int bi = INT32_MAX;
+    for (int p = tid; p < N; p += BT) {
+        const float v = S.fit[p];
+        if (v == v && better(v, p, bv, bi)) { bv = v; bi = p; }
+        if (S.fit_trace) S.fit_trace[(size_t)iter * N + p] = v;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float ov = __shfl_xor(bv, o);
+        const int oi = __shfl_xor(bi, o);
+        if (better(ov, oi, bv, bi)) { bv = ov; bi = oi; }
+    }
+    if ((tid & 63) == 0) { s_v[tid >> 6] = bv; s_i[tid >> 6] = bi; }
+    __syncthreads();
+    if (tid == 0) {
+        for (int k = 1; k < BT / 64; ++k)
+            if (better(s_v[k], s_i[k], bv, bi)) { bv = s_v[k]; bi = s_i[k]; }
+        s_best = bi == INT32_MAX ? 0 : bi;
+    }
+    __syncthreads();
+    const int best = s_best;
+
+    // personal best (strict <, main.cu:786-811)
+    for (int p = tid; p < N; p += BT) {
+        const float v = S.fit[p];
+        if (v < S.pbest_fit[p]) {
+            S.pbest_fit[p] = v;
+#pragma unroll
+            for (int d = 0; d < 7; ++d) S.pbest[d * N + p] = S.pos[d * N + p];
+        }
+    }
+    // global best (thread 0, main.cu:824-851)
+    if (tid == 0) {
+        const float fb = S.fit[best];
+        if (fb < S.gbest[7]) {
+            S.gbest[7] = fb;
+#pragma unroll
+            for (int d = 0; d < 7; ++d) S.gbest[d] = S.pos[d * N + best];
+        }
+#pragma unroll
+        for (int d = 0; d < 8; ++d) s_g[d] = S.gbest[d];
+        if (S.gbest_trace) {
+#pragma unroll
+            for (int d = 0; d < 8; ++d) S.gbest_trace[(size_t)iter * 8 + d] = s_g[d];
+        }
+        if (S.argmin_trace) S.argmin_trace[iter] = best;
+    }
+    __syncthreads();
+    for (int p = tid; p < N; p += BT) update_particle(S, p, iter, s_g);
+    if (tid == 0) *S.iter = iter + 1;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------------
+namespace qpso_launch {
+
+struct RfCfg {
+    int bt, cap;
+    const void* fn;
+};
+
+#define QPSO_RF_CFG(BT, CAP) {BT, CAP, reinterpret_cast<const void*>(&k_render_fitness<BT, CAP>)}
+static const RfCfg kRfCfgs[] = {
+    QPSO_RF_CFG(1024, 40448),  // 1 workgroup / CU, 158 KiB tile
+    QPSO_RF_CFG(512, 40448),   // 1 workgroup / CU, 8 waves
+    QPSO_RF_CFG(512, 20224),   // 2 workgroups / CU
+    QPSO_RF_CFG(256, 20224),   // 2 workgroups / CU, 4 waves each
+    QPSO_RF_CFG(256, 10112),   // 4 workgroups / CU
+};
+#undef QPSO_RF_CFG
+
+int rf_num_configs() { return (int)(sizeof(kRfCfgs) / sizeof(kRfCfgs[0])); }
+int rf_config_threads(int cfg) { return kRfCfgs[cfg].bt; }
+int rf_config_tile(int cfg) { return kRfCfgs[cfg].cap; }
+
+hipError_t init(const QpsoSolveDev* d_solves, int n_solves, hipStream_t st) {
+    if (n_solves <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_init, dim3(n_solves), dim3(256), 0, st, d_solves);
+    return hipGetLastError();
+}
+
+hipError_t render_fitness(int cfg, const QpsoSolveDev* d_solves, const QpsoItem* d_items, int n_items,
+                          hipStream_t st) {
+    if (n_items <= 0) return hipSuccess;
+    switch (cfg) {
+        case 0: hipLaunchKernelGGL((k_render_fitness<1024, 40448>), dim3(n_items), dim3(1024), 0, st, d_solves, d_items); break;
+        case 1: hipLaunchKernelGGL((k_render_fitness<512, 40448>), dim3(n_items), dim3(512), 0, st, d_solves, d_items); break;
+        case 2: hipLaunchKernelGGL((k_render_fitness<512, 20224>), dim3(n_items), dim3(512), 0, st, d_solves, d_items); break;
+        case 3: hipLaunchKernelGGL((k_render_fitness<256, 20224>), dim3(n_items), dim3(256), 0, st, d_solves, d_items); break;
+        case 4: hipLaunchKernelGGL((k_render_fitness<256, 10112>), dim3(n_items), dim3(256), 0, st, d_solves, d_items); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t pso_step(const QpsoSolveDev* d_solves, int n_solves, hipStream_t st) {
+    if (n_solves <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pso_step<256>, dim3(n_solves), dim3(256), 0, st, d_solves);
+    return hipGetLastError();
+}
+
+}  // namespace qpso_launch

@@ -1,0 +1,154 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Stage parity (rendered depth buffers, per-particle AABB and fitness for the
+same poses) and end-to-end parity of whole solves (every iteration's fitness,
+argmin, gbest, the final swarm state and the carried depth pixels), all
+bit-exact.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import qpso
+import workloads as W
+from helpers import assert_same_f32, assert_same_i32
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = qpso.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def scene0():
+    return W.config0(0)
+
+
+@pytest.fixture(scope="module")
+def scene1():
+    return W.config1(0, n_particles=48, n_iters=4)
+
+
+def sample_poses(scene, n, seed, spread_mm=25.0, wild=0.3):
+    """Poses (7 x n, metres) around the ground truth plus a share drawn like InitQPSO."""
+    rng = np.random.default_rng(seed)
+    gt = scene.objects[0][1]
+    out = np.zeros((7, n), np.float32)
+    c = scene.clusters[0]["centroid_mm"]
+    for i in range(n):
+        if rng.random() < wild:
+            t = np.array(c) * 1e-3 + rng.uniform(-0.3, 0.3, 3)
+            q = rng.normal(size=4)
+        else:
+            t = gt[:3] * 1e-3 + rng.normal(0, spread_mm * 1e-3, 3)
+            q = gt[3:] + rng.normal(0, 0.15, 4)
+        q = q / np.linalg.norm(q)
+        if q[0] < 0:
+            q = -q
+        out[:3, i], out[3:, i] = t, q
+    return out
+
+
+def oracle_eval(scene, params, poses):
+    op = W.oracle_params(O, scene, params)
+    cam = (params.cam.fx, params.cam.fy, params.cam.cx, params.cam.cy)
+    c = scene.clusters[0]
+    fits, aabbs, frames = [], [], []
+    for i in range(poses.shape[1]):
+        fr, info = O.render(scene.mesh_m(), poses[:, i], cam, c["rect"], face_cap=params.face_cap)
+        frames.append(fr.copy())
+        aabbs.append(list(info.aabb))
+        f, _ = O.fitness(info, c["rect"], c["num_points"], fr, c["depth"])
+        fits.append(f)
+    return np.array(fits, np.float32), np.array(aabbs, np.int32), np.array(frames)
+
+
+def check_eval(ctx, scene, params, poses):
+    mid = ctx.upload_mesh(scene.mesh_m())
+    try:
+        with ctx.solve(mid, scene.cluster(), params) as s:
+            fit, aabb, frames = s.eval_poses(poses, frames=True)
+    finally:
+        ctx.release_mesh(mid)
+    ofit, oaabb, oframes = oracle_eval(scene, params, poses)
+    assert_same_i32(aabb, oaabb, "aabb")
+    assert_same_i32(frames, oframes, "rendered depth buffers")
+    assert_same_f32(fit, ofit, "fitness")
+    return fit
+
+
+def test_eval_poses_config0(ctx, scene0):
+    scene, params = scene0
+    poses = sample_poses(scene, params.n_particles, seed=1)
+    fit = check_eval(ctx, scene, params, poses)
+    assert np.isfinite(fit).sum() > params.n_particles // 2
+
+
+def test_eval_poses_config1_bands(ctx, scene1):
+    # 10k faces, ROI ~220^2: the union box exceeds one tile, so rows are banded
+    scene, params = scene1
+    poses = sample_poses(scene, params.n_particles, seed=2)
+    check_eval(ctx, scene, params, poses)
+
+
+def run_solve_pair(ctx, scene, params, threads=8):
+    mid = ctx.upload_mesh(scene.mesh_m())
+    try:
+        with ctx.solve(mid, scene.cluster(), params, trace=True) as s:
+            res = s.run()
+            tr = s.trace(frames=True)
+    finally:
+        ctx.release_mesh(mid)
+    op = W.oracle_params(O, scene, params)
+    rand = O.rand_table(params.seed, params.n_particles, params.n_iters)
+    ores = O.solve(scene.mesh_m(), scene.clusters[0]["depth"], op, rand, n_threads=threads, want_frames=True)
+    return res, tr, ores
+
+
+def check_solve(res, tr, ores):
+    assert res.status == 0
+    assert_same_f32(tr["fit_trace"], ores["fit_trace"], "fitness trace")
+    assert_same_i32(tr["argmin_trace"], ores["argmin_trace"], "argmin trace")
+    assert_same_f32(tr["gbest_trace"], ores["gbest_trace"], "gbest trace")
+    assert_same_f32(tr["pos"], ores["pos"], "final positions")
+    assert_same_f32(tr["vel"], ores["vel"], "final velocities")
+    assert_same_f32(tr["pbest"], ores["pbest"], "final personal bests")
+    assert_same_f32(tr["pbest_fit"], ores["pbest_fit"], "final personal best fitness")
+    assert_same_i32(tr["frames"], ores["frames"], "carried depth pixels")
+    assert_same_f32(np.array(res.pose[:]), ores["best_pose"], "best pose")
+    assert_same_f32(np.float32(res.fit), ores["best_fit"], "best fit")
+    assert res.raster_px == ores["raster_px"]
+    assert res.visit_px == ores["visit_px"]
+
+
+def test_solve_config0_end_to_end(ctx, scene0):
+    scene, params = scene0
+    check_solve(*run_solve_pair(ctx, scene, params))
+
+
+def test_solve_config1_short(ctx, scene1):
+    scene, params = scene1
+    check_solve(*run_solve_pair(ctx, scene, params))
+
+
+@pytest.mark.parametrize("cfg", [1, 2, 3, 4])
+def test_tile_configs_agree(scene1, cfg):
+    # every (threads, tile) variant of the render+fitness kernel is bit-exact
+    scene, params = scene1
+    old = os.environ.get("QPSO_RF_CFG")
+    os.environ["QPSO_RF_CFG"] = str(cfg)
+    try:
+        with qpso.Context(0) as c:
+            poses = sample_poses(scene, params.n_particles, seed=3 + cfg)
+            check_eval(c, scene, params, poses)
+    finally:
+        if old is None:
+            del os.environ["QPSO_RF_CFG"]
+        else:
+            os.environ["QPSO_RF_CFG"] = old
