@@ -127,6 +127,17 @@ int qpso_solve_run(qpso_solve* s, qpso_result* out);
 int qpso_solve_get_trace(qpso_solve* s, qpso_trace* tr);
 int qpso_solve_destroy(qpso_solve* s);
 
+/* qpso_solve_run with HIP events recorded on the launch stream around every
+ * kernel (the reference's myTimer brackets, main.cu:1617-1733, per kernel). */
+typedef struct {
+    double ms_solve;          /* first to last event of the solve */
+    double ms_render_fitness; /* sum over the I render+fitness launches */
+    double ms_pso;            /* sum over the I argmin/pbest/gbest/update launches */
+    double ms_init;           /* InitQPSO */
+    int32_t launches;         /* render+fitness launches (= I) */
+} qpso_timing;
+int qpso_solve_run_timed(qpso_solve* s, qpso_result* out, qpso_timing* tm);
+
 /* Batched form: every listed solve runs concurrently, one fused
  * render+fitness launch per iteration over all their particles plus one PSO
  * launch, the iteration loop captured in a HIP graph.  Results are identical

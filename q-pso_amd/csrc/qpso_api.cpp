@@ -425,6 +425,45 @@ int qpso_solve_run(qpso_solve* s, qpso_result* out) {
     return read_result(s, out, ctx->stream);
 }
 
+int qpso_solve_run_timed(qpso_solve* s, qpso_result* out, qpso_timing* tm) {
+    if (!s || !tm) return QPSO_E_INVAL;
+    qpso_ctx* ctx = s->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int I = s->h.I;
+    std::vector<hipEvent_t> ev(2 + 2 * (size_t)I);
+    for (auto& e : ev) HIP_TRY(ctx, hipEventCreate(&e));
+    auto destroy = [&]() { for (auto& e : ev) hipEventDestroy(e); };
+    const QpsoItem* items = reinterpret_cast<const QpsoItem*>(s->arena + s->L.items);
+    hipError_t e = hipEventRecord(ev[0], st);
+    if (e == hipSuccess) e = qpso_launch::init(s->d_desc, 1, st);
+    if (e == hipSuccess) e = hipEventRecord(ev[1], st);
+    for (int it = 0; it < I && e == hipSuccess; ++it) {
+        e = qpso_launch::render_fitness(ctx->rf_cfg, s->d_desc, items, s->h.N, st);
+        if (e == hipSuccess) e = hipEventRecord(ev[2 + 2 * it], st);
+        if (e == hipSuccess) e = qpso_launch::pso_step(s->d_desc, 1, st);
+        if (e == hipSuccess) e = hipEventRecord(ev[3 + 2 * it], st);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) { destroy(); return set_err(ctx, QPSO_E_HIP, std::string("timed run: ") + hipGetErrorString(e)); }
+    float ms = 0;
+    std::memset(tm, 0, sizeof(*tm));
+    hipEventElapsedTime(&ms, ev[0], ev[1]);
+    tm->ms_init = ms;
+    for (int it = 0; it < I; ++it) {
+        hipEventElapsedTime(&ms, ev[1 + 2 * it], ev[2 + 2 * it]);
+        tm->ms_render_fitness += ms;
+        hipEventElapsedTime(&ms, ev[2 + 2 * it], ev[3 + 2 * it]);
+        tm->ms_pso += ms;
+    }
+    hipEventElapsedTime(&ms, ev[0], ev[1 + 2 * I]);
+    tm->ms_solve = ms;
+    tm->launches = I;
+    destroy();
+    return read_result(s, out, st);
+}
+
 int qpso_estimate_pose(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl, const qpso_params* p,
                        qpso_result* out) {
     qpso_solve* s = nullptr;

@@ -22,7 +22,8 @@ OK, E_INVAL, E_HIP, E_NOMEM, E_BUDGET, E_NOTFOUND, E_STALE, E_PENDING = 0, -1, -
 EXPORTS = [
     "qpso_params_default", "qpso_create", "qpso_destroy", "qpso_last_error", "qpso_status_str",
     "qpso_mesh_upload", "qpso_mesh_release", "qpso_estimate_pose", "qpso_solve_create",
-    "qpso_solve_run", "qpso_solve_get_trace", "qpso_solve_destroy", "qpso_solve_run_batch",
+    "qpso_solve_run", "qpso_solve_get_trace", "qpso_solve_destroy", "qpso_solve_run_timed",
+    "qpso_solve_run_batch",
     "qpso_solve_eval_poses", "qpso_server_create", "qpso_server_submit", "qpso_server_poll",
     "qpso_server_wait", "qpso_server_stats_get", "qpso_server_destroy", "qpso_solve_bytes",
     "qpso_mesh_icosphere", "qpso_mesh_uv_ellipsoid", "qpso_random_quat", "qpso_scene_render",
@@ -56,6 +57,11 @@ class Result(C.Structure):
 class Trace(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in
                 ("fit_trace", "gbest_trace", "argmin_trace", "pos", "vel", "pbest", "pbest_fit", "frames")]
+
+
+class Timing(C.Structure):
+    _fields_ = [("ms_solve", C.c_double), ("ms_render_fitness", C.c_double), ("ms_pso", C.c_double),
+                ("ms_init", C.c_double), ("launches", C.c_int32)]
 
 
 class ServerStats(C.Structure):
@@ -101,6 +107,7 @@ def lib():
             "qpso_solve_run": (I32, [P, P]),
             "qpso_solve_get_trace": (I32, [P, P]),
             "qpso_solve_destroy": (I32, [P]),
+            "qpso_solve_run_timed": (I32, [P, P, P]),
             "qpso_solve_run_batch": (I32, [P, C.c_int, P, P]),
             "qpso_solve_eval_poses": (I32, [P, C.c_int, P, P, P, P]),
             "qpso_server_create": (I32, [P, U64, P]),
@@ -242,6 +249,11 @@ class Solve:
         r = Result()
         _check(lib().qpso_solve_run(self.handle, C.byref(r)), self.ctx)
         return r
+
+    def run_timed(self):
+        r, t = Result(), Timing()
+        _check(lib().qpso_solve_run_timed(self.handle, C.byref(r), C.byref(t)), self.ctx)
+        return r, t
 
     def trace(self, frames=False):
         N, I = self.N, self.I
