@@ -68,9 +68,66 @@ struct Mt19937 {
 int rand_stride(int iters) { return std::max(512, 12 * (iters + 1)); }
 
 struct Mesh {
-    float* d = nullptr;
-    uint32_t F = 0;
+    float* d = nullptr;            // loadOBJ3 SoA [9F]
+    float* verts = nullptr;        // unique vertices [3][V]
+    QpsoFaceRef* faces = nullptr;  // interleaved raster order [F]
+    uint32_t F = 0, V = 0;
+    std::vector<uint32_t> v_prefix;  // unique vertices referenced by faces [0, f+1)
 };
+
+// Deduplicate the corners of the non-indexed SoA mesh (bitwise-equal
+// coordinates become one vertex, numbered by first use in face order, so the
+// vertices of faces [0, f) are the prefix [0, v_prefix[f-1])) and build the
+// raster table in face order: neighbouring faces in one wave write
+// neighbouring pixels, which keeps ds_min_i32 free of bank conflicts.
+void build_mesh_tables(const float* A, uint32_t F, std::vector<float>& verts, std::vector<QpsoFaceRef>& order,
+                       std::vector<uint32_t>& v_prefix) {
+    struct Key {
+        uint32_t x, y, z;
+        bool operator==(const Key& o) const { return x == o.x && y == o.y && z == o.z; }
+    };
+    struct KeyHash {
+        size_t operator()(const Key& k) const {
+            uint64_t h = k.x * 0x9E3779B97F4A7C15ull;
+            h ^= (h >> 29) + k.y * 0xBF58476D1CE4E5B9ull;
+            h ^= (h >> 31) + k.z * 0x94D049BB133111EBull;
+            return (size_t)(h ^ (h >> 32));
+        }
+    };
+    std::unordered_map<Key, int32_t, KeyHash> ids;
+    ids.reserve((size_t)F * 2);
+    std::vector<int32_t> tri((size_t)F * 3);
+    std::vector<float> xs, ys, zs;
+    v_prefix.resize(F);
+    for (uint32_t f = 0; f < F; ++f) {
+        for (int j = 0; j < 3; ++j) {
+            const float x = A[f + j * (size_t)F], y = A[3 * (size_t)F + f + j * (size_t)F],
+                        z = A[6 * (size_t)F + f + j * (size_t)F];
+            Key k;
+            std::memcpy(&k.x, &x, 4); std::memcpy(&k.y, &y, 4); std::memcpy(&k.z, &z, 4);
+            auto it = ids.find(k);
+            int32_t id;
+            if (it == ids.end()) {
+                id = (int32_t)xs.size();
+                ids.emplace(k, id);
+                xs.push_back(x); ys.push_back(y); zs.push_back(z);
+            } else {
+                id = it->second;
+            }
+            tri[(size_t)f * 3 + j] = id;
+        }
+        v_prefix[f] = (uint32_t)xs.size();
+    }
+    const size_t V = xs.size();
+    verts.resize(3 * V);
+    std::copy(xs.begin(), xs.end(), verts.begin());
+    std::copy(ys.begin(), ys.end(), verts.begin() + V);
+    std::copy(zs.begin(), zs.end(), verts.begin() + 2 * V);
+    order.clear();
+    order.reserve(F);
+    for (uint32_t f = 0; f < F; ++f)
+        order.push_back(QpsoFaceRef{tri[f * 3], tri[f * 3 + 1], tri[f * 3 + 2], (int32_t)f});
+}
 
 struct ArenaLayout {
     size_t desc, items, obs, pos, vel, pbest, pbest_fit, fit, gbest, iter, flags, counters;
@@ -282,7 +339,11 @@ int qpso_destroy(qpso_ctx* ctx) {
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
     invalidate_plan(ctx);
-    for (auto& kv : ctx->meshes) hipFree(kv.second.d);
+    for (auto& kv : ctx->meshes) {
+        hipFree(kv.second.d);
+        hipFree(kv.second.verts);
+        hipFree(kv.second.faces);
+    }
     for (auto& kv : ctx->rand_cache) hipFree(kv.second);
     hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -297,10 +358,18 @@ int qpso_mesh_upload(qpso_ctx* ctx, const float* soa, uint32_t n_faces, uint32_t
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     Mesh m;
     m.F = n_faces;
+    std::vector<float> verts;
+    std::vector<QpsoFaceRef> order;
+    build_mesh_tables(soa, n_faces, verts, order, m.v_prefix);
+    m.V = (uint32_t)(verts.size() / 3);
     HIP_TRY(ctx, hipMalloc(&m.d, sizeof(float) * 9 * (size_t)n_faces));
+    HIP_TRY(ctx, hipMalloc(&m.verts, sizeof(float) * verts.size()));
+    HIP_TRY(ctx, hipMalloc(&m.faces, sizeof(QpsoFaceRef) * order.size()));
     HIP_TRY(ctx, hipMemcpy(m.d, soa, sizeof(float) * 9 * (size_t)n_faces, hipMemcpyHostToDevice));
+    HIP_TRY(ctx, hipMemcpy(m.verts, verts.data(), sizeof(float) * verts.size(), hipMemcpyHostToDevice));
+    HIP_TRY(ctx, hipMemcpy(m.faces, order.data(), sizeof(QpsoFaceRef) * order.size(), hipMemcpyHostToDevice));
     *mesh_id = ctx->next_mesh++;
-    ctx->meshes[*mesh_id] = m;
+    ctx->meshes[*mesh_id] = std::move(m);
     return QPSO_OK;
 }
 
@@ -311,6 +380,8 @@ int qpso_mesh_release(qpso_ctx* ctx, uint32_t mesh_id) {
     if (it == ctx->meshes.end()) return QPSO_E_NOTFOUND;
     hipStreamSynchronize(ctx->stream);
     hipFree(it->second.d);
+    hipFree(it->second.verts);
+    hipFree(it->second.faces);
     ctx->meshes.erase(it);
     return QPSO_OK;
 }
@@ -350,6 +421,11 @@ int qpso_solve_create(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl, c
     h.mesh = mit->second.d;
     h.F = (int32_t)mit->second.F;
     h.F_eff = (p->face_cap > 0 && (uint32_t)p->face_cap < mit->second.F) ? p->face_cap : (int32_t)mit->second.F;
+    h.verts = mit->second.verts;
+    h.V = (int32_t)mit->second.V;
+    h.V_eff = (int32_t)mit->second.v_prefix[h.F_eff - 1];
+    h.faces = mit->second.faces;
+    h.n_faces = (int32_t)mit->second.F;
     h.fx = p->cam.fx; h.fy = p->cam.fy; h.cx = p->cam.cx; h.cy = p->cam.cy;
     h.N = N; h.I = I; h.S = S;
     h.w = p->w; h.c1 = p->c1; h.c2 = p->c2;

@@ -24,11 +24,23 @@ struct QpsoItem {  // one render+fitness workgroup: (solve index, particle index
     int32_t solve, particle;
 };
 
+// One entry of a mesh's raster table: the three unique-vertex indices of a
+// face and its index in the loadOBJ3 order.
+struct QpsoFaceRef {
+    int32_t v0, v1, v2, face;
+};
+
 struct QpsoSolveDev {
     // mesh (metres, loadOBJ3 SoA)
     const float* mesh;
     int32_t F;       // faces in the SoA (row stride)
     int32_t F_eff;   // faces rendered (face cap)
+    // deduplicated mesh: unique vertices [3][V] (x, y, z) and the raster order
+    const float* verts;
+    int32_t V;       // unique vertices (row stride of verts)
+    int32_t V_eff;   // unique vertices referenced by faces < F_eff (a prefix)
+    const QpsoFaceRef* faces;
+    int32_t n_faces; // entries in `faces` (= F)
     // camera
     double fx, fy, cx, cy;
     // PSO

@@ -93,18 +93,36 @@ __global__ __launch_bounds__(256) void k_init(const QpsoSolveDev* __restrict__ s
     }
 }
 
+
 // ------------------------------------------------------------------------
 // fused render + AABB + fitness
 // ------------------------------------------------------------------------
+// Number of times the fitness scan (main.cu:632-733) visits the pixel at
+// offset (xb, k) of the union box: the row index of t is the fp32 product
+// trunc(t * rcp(W)), so for t = kW it falls one row short when that product
+// rounds below k; (0, k) is then skipped by t = kW and visited by t = (k+1)W.
+struct Visits {
+    int32_t W, H, T;
+    float rW;
+    __device__ __forceinline__ bool short_row(int32_t k) const {
+        return k > 0 && row_of(k * W, rW) == k - 1;
+    }
+    __device__ __forceinline__ int32_t at(int32_t xb, int32_t k) const {
+        if (xb != 0) return (k < H || k * W + xb < T) ? 1 : 0;
+        return ((k * W < T && !short_row(k)) ? 1 : 0) + (((k + 1) * W < T && short_row(k + 1)) ? 1 : 0);
+    }
+};
+
 template <int BT, int CAP>
 __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __restrict__ solves,
                                                       const QpsoItem* __restrict__ items) {
     constexpr int SPT = 2048 / BT;  // fitness slots per thread
-    __shared__ int32_t tile[CAP];
-    __shared__ int32_t s_box[4];
+    __shared__ int32_t lds[CAP];    // [vertex cache X | Y | Z] [depth tile]
+    __shared__ int32_t s_box[4];    // rendered AABB: minX, maxX, minY, maxY
+    __shared__ int32_t s_stale[4];  // carried pixels inside the union box: x0, x1, y0, y1 (inclusive)
     __shared__ int32_t s_degen;
     __shared__ int32_t s_nstale;
-    __shared__ uint32_t s_cnt[3];  // seg, tr, n_rendered
+    __shared__ uint32_t s_cnt[3];   // seg, tr, n_rendered
     __shared__ unsigned long long s_px[2];
 
     const int tid = threadIdx.x;
@@ -125,24 +143,57 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     if (tid == 0) {
         s_box[0] = QPSO_AABB_MIN_SENT; s_box[1] = QPSO_AABB_MAX_SENT;
         s_box[2] = QPSO_AABB_MIN_SENT; s_box[3] = QPSO_AABB_MAX_SENT;
+        s_stale[0] = INT32_MAX; s_stale[1] = INT32_MIN;
+        s_stale[2] = INT32_MAX; s_stale[3] = INT32_MIN;
         s_degen = 0; s_nstale = 0;
         s_cnt[0] = s_cnt[1] = s_cnt[2] = 0;
         s_px[0] = s_px[1] = 0;
     }
     __syncthreads();
 
+    // The projected unique vertices stay in LDS when they take at most half of
+    // it; otherwise every pass projects the three corners of each face again.
+    const int Ve = S.V_eff;
+    const bool cached = 3 * Ve <= CAP / 2;
+    int32_t* __restrict__ vX = lds;
+    int32_t* __restrict__ vY = lds + Ve;
+    float* __restrict__ vZ = reinterpret_cast<float*>(lds + 2 * Ve);
+    int32_t* __restrict__ tile = cached ? lds + 3 * Ve : lds;
+    const int32_t tcap = cached ? CAP - 3 * Ve : CAP;
+
     // ---- pass A: per-face bbox and AABB over every face (main.h:1340-1353,
-    //      AABBreduction main.h:1190-1304, final reduction main.cu:423-546)
+    //      AABBreduction main.h:1190-1304, final reduction main.cu:423-546).
+    //      (m + 15) >> 4 is monotone for |m| <= 2^30, so with the vertex cache
+    //      the face-bbox extremes are those of the vertex coordinates.
     {
         int bx0 = QPSO_AABB_MIN_SENT, bx1 = QPSO_AABB_MAX_SENT;
         int by0 = QPSO_AABB_MIN_SENT, by1 = QPSO_AABB_MAX_SENT;
         int dg = 0;
-        for (int f = tid; f < Fe; f += BT) {
-            Face fc;
-            face_project(mesh, F, f, pose, cam, fc);
-            bx0 = min(bx0, fc.minx); bx1 = max(bx1, fc.maxx);
-            by0 = min(by0, fc.miny); by1 = max(by1, fc.maxy);
-            dg |= fc.degen ? 1 : 0;
+        if (cached) {
+            const float* __restrict__ wx = S.verts;
+            const float* __restrict__ wy = wx + S.V;
+            const float* __restrict__ wz = wy + S.V;
+            int32_t xl = INT32_MAX, xh = INT32_MIN, yl = INT32_MAX, yh = INT32_MIN;
+            for (int v = tid; v < Ve; v += BT) {
+                int32_t X, Y;
+                float Z;
+                dg |= vertex_fixed(pose, wx[v], wy[v], wz[v], cam, X, Y, Z) ? 1 : 0;
+                vX[v] = X; vY[v] = Y; vZ[v] = Z;
+                xl = min(xl, X); xh = max(xh, X);
+                yl = min(yl, Y); yh = max(yh, Y);
+            }
+            if (xl <= xh) {
+                bx0 = min(bx0, wadd(xl, 15) >> 4); bx1 = max(bx1, wadd(xh, 15) >> 4);
+                by0 = min(by0, wadd(yl, 15) >> 4); by1 = max(by1, wadd(yh, 15) >> 4);
+            }
+        } else {
+            for (int f = tid; f < Fe; f += BT) {
+                Face fc;
+                face_project(mesh, F, f, pose, cam, fc);
+                bx0 = min(bx0, fc.minx); bx1 = max(bx1, fc.maxx);
+                by0 = min(by0, fc.miny); by1 = max(by1, fc.maxy);
+                dg |= fc.degen ? 1 : 0;
+            }
         }
         bx0 = wave_min(bx0); bx1 = wave_max(bx1);
         by0 = wave_min(by0); by1 = wave_max(by1);
@@ -155,8 +206,9 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     }
     __syncthreads();
 
-    const int32_t minX = min(s_box[0], S.rect[0]), maxX = max(s_box[1], S.rect[2]);
-    const int32_t minY = min(s_box[2], S.rect[1]), maxY = max(s_box[3], S.rect[3]);
+    const int32_t ax0 = s_box[0], ax1 = s_box[1], ay0 = s_box[2], ay1 = s_box[3];
+    const int32_t minX = min(ax0, S.rect[0]), maxX = max(ax1, S.rect[2]);
+    const int32_t minY = min(ay0, S.rect[1]), maxY = max(ay1, S.rect[3]);
     const long long W64 = (long long)maxX - minX, H64 = (long long)maxY - minY;
     const bool degen = s_degen != 0 || W64 <= 0 || H64 <= 0 || W64 * H64 > (long long)QPSO_AREA_LIMIT;
 
@@ -182,33 +234,74 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     const int32_t W = (int32_t)W64, H = (int32_t)H64;
     const int32_t A = W * H;
     const int32_t T = ((A + 1023) >> 10) << 10;  // NumIters * 1024 (main.cu:621-623)
-    const float rW = rcp((float)W);
-    // in-frame part of the union box: columns [tx0, tx1), rows [ty0, ty1]
-    const int32_t tx0 = max(minX, 0), tx1 = min(maxX, kCols);
-    const int32_t ty0 = max(minY, 0), ty1 = min(maxY, kRows - 1);
-    const int32_t TW = tx1 - tx0;
-    const int32_t BH = CAP / TW;
+    const Visits vis{W, H, T, rcp((float)W)};
     const int32_t rx0 = S.rect[0], ry0 = S.rect[1], rx1 = S.rect[2], ry1 = S.rect[3];
     const int32_t rw = rx1 - rx0;
     const uint16_t* __restrict__ obs = S.obs;
 
+    // carried pixels inside the union box join the tile region
+    {
+        int32_t sx0 = INT32_MAX, sx1 = INT32_MIN, sy0 = INT32_MAX, sy1 = INT32_MIN;
+        for (int k = tid; k < n_old; k += BT) {
+            const QpsoStale e = old[k];
+            const int32_t ex = e.pos & 0xffff, ey = e.pos >> 16;
+            if (ex >= minX && ex < maxX && ey >= minY && ey <= maxY) {
+                sx0 = min(sx0, ex); sx1 = max(sx1, ex);
+                sy0 = min(sy0, ey); sy1 = max(sy1, ey);
+            }
+        }
+        if (n_old > 0) {
+            sx0 = wave_min(sx0); sx1 = wave_max(sx1);
+            sy0 = wave_min(sy0); sy1 = wave_max(sy1);
+            if ((tid & 63) == 0 && sx0 <= sx1) {
+                atomicMin(&s_stale[0], sx0); atomicMax(&s_stale[1], sx1);
+                atomicMin(&s_stale[2], sy0); atomicMax(&s_stale[3], sy1);
+            }
+        }
+    }
+    __syncthreads();
+    // Tile region: rendered AABB plus those carried pixels, inside the frame.
+    // Every depth value other than 60000 in the union box lies in it.
+    int32_t tx0 = ax0, tx1 = ax1, ty0 = ay0, ty1 = ay1;  // [x0, x1) x [y0, y1)
+    if (s_stale[0] <= s_stale[1]) {
+        tx0 = min(tx0, s_stale[0]); tx1 = max(tx1, s_stale[1] + 1);
+        ty0 = min(ty0, s_stale[2]); ty1 = max(ty1, s_stale[3] + 1);
+    }
+    tx0 = max(tx0, 0); tx1 = min(tx1, kCols);
+    ty0 = max(ty0, 0); ty1 = min(ty1, kRows);
+    const bool has_tile = tx0 < tx1 && ty0 < ty1;
+    const int32_t TW = has_tile ? tx1 - tx0 : 1;
+    const int32_t BH = tcap / TW;  // >= 1: TW <= 640 and tcap >= CAP / 2
+
     float acc_e[SPT];
-    uint32_t acc_s[SPT], acc_t[SPT];
 #pragma unroll
-    for (int j = 0; j < SPT; ++j) { acc_e[j] = 0.0f; acc_s[j] = 0; acc_t[j] = 0; }
-    uint32_t n_rend = 0;
-    unsigned long long px_r = 0, px_f = 0;
+    for (int j = 0; j < SPT; ++j) acc_e[j] = 0.0f;
+    uint32_t n_seg = 0, n_tr = 0, n_rend = 0;
+    unsigned long long px_r = 0;
 
     auto push_stale = [&](int32_t x, int32_t y, int32_t v) {
         const int k = atomicAdd(&s_nstale, 1);
         if (k < cap) nw[k] = QpsoStale{(y << 16) | x, v};
         else atomicOr(S.flags, QPSO_FLAG_STALE_OVERFLOW);
     };
-    // y(kW) == k - 1: the fp32 row index of t = kW falls one row short (main.cu:645)
-    auto short_row = [&](int32_t k) -> bool { return k > 0 && row_of(k * W, rW) == k - 1; };
+    auto in_tile = [&](int32_t x, int32_t y) { return x >= tx0 && x < tx1 && y >= ty0 && y < ty1; };
 
-    for (int32_t b0 = ty0; b0 <= ty1; b0 += BH) {
-        const int32_t b1 = min(b0 + BH, ty1 + 1);  // band rows [b0, b1)
+    // ---- cluster pixels outside the tile region: depth buffer reads 60000
+    //      there, so only "observed but not rendered" counts (order-free)
+    {
+        const int32_t dq = BT / rw, dr = BT % rw;
+        int32_t y = ry0 + tid / rw, x = rx0 + tid % rw;
+        for (int i = tid; i < rw * (ry1 - ry0); i += BT) {
+            if (!in_tile(x, y) && obs[i] != 0) n_seg += (uint32_t)vis.at(x - minX, y - minY);
+            x += dr; y += dq;
+            if (x >= rx1) { x -= rw; ++y; }
+        }
+    }
+
+    uint64_t fmask = ~0ull;  // bit k: face entry tid + k*BT still reaches later bands
+    const bool use_mask = S.n_faces <= 64 * BT;
+    for (int32_t b0 = ty0; has_tile && b0 < ty1; b0 += BH) {
+        const int32_t b1 = min(b0 + BH, ty1);  // band rows [b0, b1)
         const int32_t nt = (b1 - b0) * TW;
         for (int i = tid; i < nt; i += BT) tile[i] = QPSO_EMPTY;
         __syncthreads();
@@ -220,29 +313,56 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         }
         __syncthreads();
 
-        // ---- raster: bbox fill with ds_min_i32 (main.h:1355-1488)
-        for (int f = tid; f < Fe; f += BT) {
-            Face fc;
-            face_project(mesh, F, f, pose, cam, fc);
-            if (fc.maxy <= b0 || fc.miny >= b1) continue;
-            Setup su;
-            if (!face_setup(fc, su)) continue;
-            const int32_t x0 = max(fc.minx, tx0), x1 = min(fc.maxx, tx1);
-            const int32_t y0 = max(fc.miny, b0), y1 = min(fc.maxy, b1);
-            if (x0 >= x1 || y0 >= y1) continue;
-            px_r += (unsigned long long)((x1 - x0) * (y1 - y0));
-            for (int32_t y = y0; y < y1; ++y) {
-                const int32_t cy2 = wadd(su.CY2, wmul(y - fc.miny, su.FDX23));
-                const int32_t cy3 = wadd(su.CY3, wmul(y - fc.miny, su.FDX31));
-                int32_t CX2 = wsub(cy2, wmul(x0 - fc.minx, su.FDY23));
-                int32_t CX3 = wsub(cy3, wmul(x0 - fc.minx, su.FDY31));
-                int32_t* row = tile + (y - b0) * TW - tx0;
-                for (int32_t x = x0; x < x1; ++x) {
-                    atomicMin(&row[x], pixel_depth(CX2, CX3, su));
-                    CX2 = wsub(CX2, su.FDY23);
-                    CX3 = wsub(CX3, su.FDY31);
+        // ---- raster: bbox fill with ds_min_i32 (main.h:1355-1488).  Each lane
+        //      walks its face's clipped bbox as one flat pixel loop (a nested
+        //      row/column loop would diverge as max rows x max columns).
+        int k = 0;
+        QpsoFaceRef nxt = tid < S.n_faces ? S.faces[tid] : QpsoFaceRef{0, 0, 0, INT32_MAX};
+        for (int i = tid; i < S.n_faces; i += BT, ++k) {
+            const QpsoFaceRef e = nxt;  // prefetched one iteration ahead
+            if (i + BT < S.n_faces) nxt = S.faces[i + BT];
+            if (use_mask && !((fmask >> k) & 1ull)) continue;
+            bool later = false;
+            if (e.face < Fe) {
+                Face fc;
+                if (cached) {
+                    fc.X[0] = vX[e.v0]; fc.Y[0] = vY[e.v0]; fc.Z[0] = vZ[e.v0];
+                    fc.X[1] = vX[e.v1]; fc.Y[1] = vY[e.v1]; fc.Z[1] = vZ[e.v1];
+                    fc.X[2] = vX[e.v2]; fc.Y[2] = vY[e.v2]; fc.Z[2] = vZ[e.v2];
+                    face_bbox(fc);
+                } else {
+                    face_project(mesh, F, e.face, pose, cam, fc);
+                }
+                Setup su;
+                const int32_t x0 = max(fc.minx, tx0), x1 = min(fc.maxx, tx1);
+                if (x0 < x1 && fc.miny < ty1 && fc.maxy > b0 && face_setup(fc, su)) {
+                    later = fc.maxy > b1;
+                    const int32_t y0 = max(fc.miny, b0), y1 = min(fc.maxy, b1);
+                    if (y0 < y1) {
+                        const int32_t w = x1 - x0, n = w * (y1 - y0);
+                        px_r += (unsigned long long)n;
+                        // edge values at (x0, y0); per pixel CX -= FDY; from the last
+                        // pixel of a row to the first of the next, CX += FDX + (w-1) FDY
+                        int32_t CX2 = wsub(wadd(su.CY2, wmul(y0 - fc.miny, su.FDX23)), wmul(x0 - fc.minx, su.FDY23));
+                        int32_t CX3 = wsub(wadd(su.CY3, wmul(y0 - fc.miny, su.FDX31)), wmul(x0 - fc.minx, su.FDY31));
+                        const int32_t step2 = wsub(0, su.FDY23), step3 = wsub(0, su.FDY31);
+                        const int32_t wrap2 = wadd(su.FDX23, wmul(w - 1, su.FDY23));
+                        const int32_t wrap3 = wadd(su.FDX31, wmul(w - 1, su.FDY31));
+                        const int32_t awrap = TW - w + 1;
+                        int32_t* addr = tile + (y0 - b0) * TW + (x0 - tx0);
+                        int32_t c = w;
+                        for (int32_t m = 0; m < n; ++m) {
+                            atomicMin(addr, pixel_depth(CX2, CX3, su));
+                            const bool wrap = --c == 0;
+                            CX2 = wadd(CX2, wrap ? wrap2 : step2);
+                            CX3 = wadd(CX3, wrap ? wrap3 : step3);
+                            addr += wrap ? awrap : 1;
+                            c = wrap ? w : c;
+                        }
+                    }
                 }
             }
+            if (use_mask && !later) fmask &= ~(1ull << k);
         }
         __syncthreads();
 
@@ -251,30 +371,48 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             for (int i = tid; i < nt; i += BT) fr[(b0 + i / TW) * kCols + tx0 + i % TW] = tile[i];
         }
 
-        // ---- fitness over the t whose row falls in this band (main.cu:632-733)
-        const int32_t k0 = b0 - minY, k1 = b1 - minY;
-        const int32_t tlo = min(k0 * W + (short_row(k0) ? 1 : 0), T);
-        const int32_t thi = min(k1 * W + (short_row(k1) ? 1 : 0), T);
-        for (int32_t base = (tlo / 2048) * 2048; base < thi; base += 2048) {
+        // ---- counters over the band's tile pixels, weighted by visits (order-free)
+        const int32_t dq = BT / TW, dr = BT % TW;
+        int32_t cy = b0 + tid / TW, cx = tx0 + tid % TW;
+        for (int i = tid; i < nt; i += BT, cx += dr, cy += dq) {
+            if (cx >= tx1) { cx -= TW; ++cy; }
+            const int32_t y = cy, x = cx;
+            const int32_t zr = tile[i];
+            const bool in_rect = x >= rx0 && x < rx1 && y >= ry0 && y < ry1;
+            const int32_t zk = in_rect ? (int32_t)obs[(y - ry0) * rw + (x - rx0)] : 0;
+            if (zr == QPSO_EMPTY && zk == 0) continue;
+            const uint32_t w = (uint32_t)vis.at(x - minX, y - minY);
+            if (zr != QPSO_EMPTY) {
+                n_rend += w;
+                if (zk == 0) n_tr += w;
+            } else {
+                n_seg += w;
+            }
+        }
+
+        // ---- squared error: the only order-sensitive sum.  Slot t & 2047
+        //      accumulates in increasing t (main.cu:693-717); contributions come
+        //      only from tile pixels inside the cluster rect, so scan the t whose
+        //      row lies in this band and whose pixel lies in that intersection.
+        const int32_t qx0 = max(tx0, rx0), qx1 = min(tx1, rx1);
+        const int32_t qy0 = max(b0, ry0), qy1 = min(b1, ry1);
+        if (qx0 < qx1 && qy0 < qy1) {
+            const int32_t k0 = qy0 - minY, k1 = qy1 - minY;
+            const int32_t tlo = min(k0 * W + (qx0 - minX), T);
+            const int32_t thi = min(k1 * W + 1, T);
+            for (int32_t base = (tlo / 2048) * 2048; base < thi; base += 2048) {
 #pragma unroll
-            for (int j = 0; j < SPT; ++j) {
-                const int32_t t = base + j * BT + tid;  // slot t & 2047 = j*BT + tid
-                if (t < tlo || t >= thi) continue;
-                const int32_t y = row_of(t, rW);
-                int32_t x = t - y * W;
-                if (x >= W) x -= W;  // t = kW mapped to row k-1
-                const int32_t px = minX + x, py = minY + y;
-                if (px < tx0 || px >= tx1) continue;  // off-frame column: empty
-                const int32_t zr = tile[(py - b0) * TW + px - tx0];
-                const int32_t zk = (px >= rx0 && px < rx1 && py >= ry0 && py < ry1)
-                                       ? (int32_t)obs[(py - ry0) * rw + (px - rx0)] : 0;
-                ++px_f;
-                if (zk != 0 && zr == QPSO_EMPTY) acc_s[j] += 1;
-                if (zr != QPSO_EMPTY) {
-                    ++n_rend;
-                    if (zk == 0) {
-                        acc_t[j] += 1;
-                    } else {
+                for (int j = 0; j < SPT; ++j) {
+                    const int32_t t = base + j * BT + tid;  // slot t & 2047 = j*BT + tid
+                    if (t < tlo || t >= thi) continue;
+                    const int32_t y = row_of(t, vis.rW);
+                    int32_t x = t - y * W;
+                    if (x >= W) x -= W;  // t = kW mapped to row k-1
+                    const int32_t px = minX + x, py = minY + y;
+                    if (px < qx0 || px >= qx1 || py < qy0 || py >= qy1) continue;
+                    const int32_t zr = tile[(py - b0) * TW + px - tx0];
+                    const int32_t zk = (int32_t)obs[(py - ry0) * rw + (px - rx0)];
+                    if (zr != QPSO_EMPTY && zk != 0) {
                         const float e = (float)wsub(zr, zk) * 0.001f;
                         acc_e[j] = __builtin_fmaf(e, e, acc_e[j]);
                     }
@@ -282,12 +420,11 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             }
         }
 
-        // ---- pixels of this band the scan never visits keep their depth
-        if (minX >= tx0) {  // x-offset 0 column is in frame
+        // ---- tile pixels the scan never visits keep their depth
+        if (minX >= tx0 && minX < tx1) {  // column x-offset 0
             for (int32_t py = b0 + tid; py < b1; py += BT) {
-                const int32_t k = py - minY;
-                const bool vis = (k * W < T && !short_row(k)) || ((k + 1) * W < T && short_row(k + 1));
-                if (!vis) {
+                if (py > maxY) continue;
+                if (vis.at(0, py - minY) == 0) {
                     const int32_t v = tile[(py - b0) * TW + minX - tx0];
                     if (v != QPSO_EMPTY) push_stale(minX, py, v);
                 }
@@ -303,11 +440,27 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         __syncthreads();
     }
 
-    // carried pixels outside this iteration's box stay as they are
+    // carried pixels outside this iteration's union box stay as they are
     for (int k = tid; k < n_old; k += BT) {
         const QpsoStale e = old[k];
         const int32_t ex = e.pos & 0xffff, ey = e.pos >> 16;
-        if (!(ey >= ty0 && ey <= ty1 && ex >= tx0 && ex < tx1)) push_stale(ex, ey, e.val);
+        if (!(ex >= minX && ex < maxX && ey >= minY && ey <= maxY)) push_stale(ex, ey, e.val);
+    }
+
+    // ---- in-frame visits of the scan (work counter P_f), one row per thread
+    unsigned long long px_f = 0;
+    {
+        const int32_t fx0 = max(minX, 0), fx1 = min(maxX, kCols);
+        const int32_t fy0 = max(minY, 0), fy1 = min(maxY, kRows - 1);
+        const bool col0 = minX >= 0;
+        for (int32_t py = fy0 + tid; py <= fy1; py += BT) {
+            const int32_t k = py - minY;
+            int32_t n;
+            if (k < H) n = fx1 - fx0 - (col0 ? 1 : 0);
+            else n = max(0, min(fx1 - minX, T - H * W) - max(fx0 - minX, 1));
+            if (col0) n += vis.at(0, k);
+            px_f += (unsigned long long)n;
+        }
     }
 
     // ---- reductions (FitnessReductionBallot, main.h:1522-1573): halving tree
@@ -317,11 +470,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
 #pragma unroll
         for (int j = 0; j < stride / BT; ++j) acc_e[j] = acc_e[j] + acc_e[j + stride / BT];
     }
-    uint32_t cs = 0, ct = 0;
-#pragma unroll
-    for (int j = 0; j < SPT; ++j) { cs += acc_s[j]; ct += acc_t[j]; }
-    cs = wave_sum(cs); ct = wave_sum(ct);
-    const uint32_t cr = wave_sum(n_rend);
+    const uint32_t cs = wave_sum(n_seg), ct = wave_sum(n_tr), cr = wave_sum(n_rend);
     if ((tid & 63) == 0) {
         atomicAdd(&s_cnt[0], cs); atomicAdd(&s_cnt[1], ct); atomicAdd(&s_cnt[2], cr);
     }

@@ -23,11 +23,15 @@ QD int32_t wmul(int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint32_t
 QD int32_t wshl(int32_t a, int s) { return (int32_t)((uint32_t)a << s); }
 
 // __float2int_rn: round half to even, saturating, NaN -> 0 (main.h:1331-1338).
+// v_rndne_f32 then v_cvt_i32_f32: the CDNA conversion clamps out-of-range
+// values to INT32_MIN/INT32_MAX and converts NaN to 0, which is exactly the
+// saturating semantics above (a C cast would be undefined out of range, so the
+// conversion is issued directly; it is a plain VALU op).
 QD int32_t rint_sat(float x) {
-    if (x != x) return 0;
-    if (x >= 2147483648.0f) return INT32_MAX;
-    if (x <= -2147483648.0f) return INT32_MIN;
-    return (int32_t)__builtin_rintf(x);
+    const float r = __builtin_rintf(x);
+    int32_t v;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(v) : "v"(r));
+    return v;
 }
 
 QD float rcp(float x) { return 1.0f / x; }  // __frcp_rn
@@ -215,6 +219,23 @@ struct Face {
     bool degen;
 };
 
+QD void face_bbox(Face& fc) {
+    fc.minx = wadd(min(min(fc.X[0], fc.X[1]), fc.X[2]), 15) >> 4;
+    fc.maxx = wadd(max(max(fc.X[0], fc.X[1]), fc.X[2]), 15) >> 4;
+    fc.miny = wadd(min(min(fc.Y[0], fc.Y[1]), fc.Y[2]), 15) >> 4;
+    fc.maxy = wadd(max(max(fc.Y[0], fc.Y[1]), fc.Y[2]), 15) >> 4;
+}
+
+// Projected vertex in 28.4 fixed point; `degen` if outside the +-2^30 range.
+QD bool vertex_fixed(const float pose[7], float vx, float vy, float vz, const Cam& c,
+                     int32_t& X, int32_t& Y, float& Z) {
+    float u, v;
+    project(pose, vx, vy, vz, c, u, v, Z);
+    X = rint_sat(16.0f * u);
+    Y = rint_sat(16.0f * v);
+    return (X > QPSO_FXP_LIMIT) | (X < -QPSO_FXP_LIMIT) | (Y > QPSO_FXP_LIMIT) | (Y < -QPSO_FXP_LIMIT);
+}
+
 QD void face_project(const float* __restrict__ mesh, int F, int f, const float pose[7],
                      const Cam& c, Face& fc) {
     fc.degen = false;
@@ -231,10 +252,7 @@ QD void face_project(const float* __restrict__ mesh, int F, int f, const float p
         fc.degen |= (fc.X[j] > QPSO_FXP_LIMIT) | (fc.X[j] < -QPSO_FXP_LIMIT) |
                     (fc.Y[j] > QPSO_FXP_LIMIT) | (fc.Y[j] < -QPSO_FXP_LIMIT);
     }
-    fc.minx = wadd(min(min(fc.X[0], fc.X[1]), fc.X[2]), 15) >> 4;
-    fc.maxx = wadd(max(max(fc.X[0], fc.X[1]), fc.X[2]), 15) >> 4;
-    fc.miny = wadd(min(min(fc.Y[0], fc.Y[1]), fc.Y[2]), 15) >> 4;
-    fc.maxy = wadd(max(max(fc.Y[0], fc.Y[1]), fc.Y[2]), 15) >> 4;
+    face_bbox(fc);
 }
 
 // Raster set-up (main.h:1355-1419): edge functions E23 and E31 at the bbox
