@@ -1,0 +1,186 @@
+"""CPU tests of the host side: the C ABI library loads and exports every
+symbol the headers declare, the reference defaults, the synthetic scene and
+cluster-ROI provider, and the file formats (OBJ, .dpt, bestpose.csv)."""
+import ctypes as C
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+import qpso
+import workloads as W
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(REPO, "include", "*.h")):
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b(qpso_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = C.CDLL(qpso.LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 30
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+    assert names == set(qpso.EXPORTS), names ^ set(qpso.EXPORTS)
+
+
+def test_params_default_are_reference_constants():
+    p = qpso.default_params()
+    assert (p.n_particles, p.n_iters, p.w, p.c1, p.c2) == (1024, 20, 0.3, 1.0, 1.0)
+    assert abs(p.t_bound - 0.3) < 1e-7 and abs(p.t_bound_z - 0.3) < 1e-7 and p.seed == 0
+    assert (p.cam.fx, p.cam.fy, p.cam.cx, p.cam.cy) == (572.41140, 573.57043, 325.26110, 242.04899)
+
+
+def test_status_strings():
+    L = qpso.lib()
+    for code in (0, -1, -2, -3, -4, -5, -6, -7):
+        assert L.qpso_status_str(code)
+    assert L.qpso_status_str(-99) == b"unknown status"
+
+
+def test_create_without_gpu_fails_loudly():
+    import subprocess
+    probe = subprocess.run(["python3", "-c", "import torch; print(torch.cuda.device_count())"],
+                           capture_output=True, text=True)
+    if probe.returncode == 0 and probe.stdout.strip() not in ("", "0"):
+        pytest.skip("a GPU is present")
+    h = C.c_void_p()
+    assert qpso.lib().qpso_create(0, C.byref(h)) == qpso.E_HIP and not h.value
+
+
+def test_solve_bytes_grows_with_swarm():
+    p = qpso.make_params(128, 60)
+    a = qpso.lib().qpso_solve_bytes(C.byref(p), (C.c_int32 * 4)(0, 0, 100, 100))
+    p.n_particles = 256
+    b = qpso.lib().qpso_solve_bytes(C.byref(p), (C.c_int32 * 4)(0, 0, 100, 100))
+    assert 0 < a < b
+
+
+def soa_faces(soa):
+    F = soa.size // 9
+    return soa.reshape(3, 3, F)  # [xyz][slot][face]
+
+
+@pytest.mark.parametrize("subdiv", [0, 1, 3])
+def test_icosphere_closed_and_outward(subdiv):
+    m = qpso.mesh_icosphere(subdiv, 50.0, 35.0, 25.0)
+    F = m.size // 9
+    assert F == 20 * 4 ** subdiv
+    V = soa_faces(m).astype(np.float64)
+    v0, v1, v2 = V[:, 0].T, V[:, 1].T, V[:, 2].T
+    n = np.cross(v1 - v0, v2 - v0)
+    c = (v0 + v1 + v2) / 3
+    assert np.all(np.einsum("ij,ij->i", n, c) > 0)  # outward winding = front-facing rule
+    # closed: every directed edge has its reverse
+    e = {}
+    for f in range(F):
+        for a, b in ((0, 1), (1, 2), (2, 0)):
+            e[(tuple(V[:, a, f]), tuple(V[:, b, f]))] = 1
+    assert all((b, a) in e for (a, b) in e)
+
+
+def test_uv_ellipsoid_face_count_and_boxes():
+    assert qpso.lib().qpso_mesh_uv_ellipsoid(80, 65, 1, 1, 1, 0, 10, 20, 0, None) == 10240
+    m = qpso.mesh_uv_ellipsoid(80, 65, 160.0, 115.0, 85.0, 8, 10.0, 20.0, 7)
+    assert m.size // 9 == 10240 + 96
+    m2 = qpso.mesh_uv_ellipsoid(80, 65, 160.0, 115.0, 85.0, 8, 10.0, 20.0, 7)
+    assert np.array_equal(m, m2)
+
+
+def test_scene_plane_only_and_noise():
+    cam = W.CAM
+    depth, labels = qpso.scene_render([], cam, 1100.0, 0.0, 0.0, 1)
+    assert np.all(depth == 1100) and np.all(labels == -1)
+    depth, _ = qpso.scene_render([], cam, 1100.0, 2.0, 0.05, 1)
+    holes = (depth == 0).mean()
+    assert 0.04 < holes < 0.06
+    d = depth[depth > 0].astype(np.float64)
+    assert abs(d.mean() - 1100) < 0.05 and 1.8 < d.std() < 2.2
+
+
+def test_scene_sphere_depth_and_cluster_roi():
+    cam = W.CAM
+    mesh = qpso.mesh_icosphere(4, 50.0, 50.0, 50.0)
+    pose = np.array([0, 0, 800, 1, 0, 0, 0], np.float32)
+    depth, labels = qpso.scene_render([(mesh, pose)], cam, 1100.0, 0.0, 0.0, 0)
+    # pixel-centre ray through the image centre hits the sphere front at ~750 mm
+    assert abs(int(depth[240, 320]) - 750) <= 1
+    c = qpso.scene_cluster(depth, labels, 0, cam)
+    r = c["rect"]
+    # silhouette radius ~ f * R / sqrt(d^2 - R^2)
+    rad = 525 * 50 / np.sqrt(800 ** 2 - 50 ** 2)
+    assert abs((r[2] - r[0]) / 2 - rad) < 2 and abs((r[3] - r[1]) / 2 - rad) < 2
+    assert c["num_points"] == (labels == 0).sum()
+    assert np.all(c["depth"][labels != 0] == 0) and np.all(c["depth"][labels == 0] == depth[labels == 0])
+    assert abs(c["centroid_mm"][2] - 765) < 10 and abs(c["centroid_mm"][0]) < 2
+
+
+def test_workload_configs_build():
+    s0, p0 = W.config0(0)
+    assert s0.objects[0][0].size // 9 == 1280 and p0.n_particles == 32 and p0.n_iters == 30
+    assert s0.clusters[0] is not None
+    s1, p1 = W.config1(0)
+    assert s1.objects[0][0].size // 9 == 10336 and p1.n_particles == 256 and p1.n_iters == 100
+
+
+def test_obj_reader_layout(tmp_path):
+    path = tmp_path / "quad.obj"
+    path.write_text("o first\nv 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nf 1 2 3 4\nf 1/1/1 3//2 4\n"
+                    "o second\nv 5 5 5\nf 5 5 5\n")
+    soa = C.c_void_p()
+    nf = C.c_uint32()
+    assert qpso.lib().qpso_read_obj(str(path).encode(), C.byref(soa), C.byref(nf)) == 0
+    F = nf.value
+    a = np.ctypeslib.as_array(C.cast(soa, C.POINTER(C.c_float)), shape=(9 * F,)).copy()
+    qpso.lib().qpso_free(soa)
+    assert F == 3  # quad fan-split into 2, plus 1 triangle; second object ignored
+    V = a.reshape(3, 3, F)
+    assert list(V[:, :, 0].T.reshape(-1)) == [0, 0, 0, 1, 0, 0, 1, 1, 0]
+    assert list(V[:, :, 1].T.reshape(-1)) == [0, 0, 0, 1, 1, 0, 0, 1, 0]
+
+
+def test_dpt_round_trip(tmp_path):
+    path = str(tmp_path / "d.dpt").encode()
+    d = (np.arange(480 * 640) % 65536).astype(np.uint16).reshape(480, 640)
+    assert qpso.lib().qpso_write_dpt(path, d.ctypes.data, 480, 640) == 0
+    raw = open(path, "rb").read()
+    assert np.frombuffer(raw[:8], np.int32).tolist() == [480, 640] and len(raw) == 8 + 2 * 480 * 640
+    out, r, c = C.c_void_p(), C.c_int32(), C.c_int32()
+    assert qpso.lib().qpso_read_dpt(path, C.byref(out), C.byref(r), C.byref(c)) == 0
+    got = np.ctypeslib.as_array(C.cast(out, C.POINTER(C.c_uint16)), shape=(480 * 640,)).copy()
+    qpso.lib().qpso_free(out)
+    assert (r.value, c.value) == (480, 640) and np.array_equal(got.reshape(480, 640), d)
+
+
+def test_bestpose_csv_format(tmp_path):
+    path = str(tmp_path / "x-0-bestpose.csv")
+    pose = np.array([0.1, -0.2, 0.8, 1, 0, 0, 0], np.float32)
+    assert qpso.lib().qpso_write_bestpose(path.encode(), C.c_float(0.5), pose.ctypes.data) == 0
+    assert open(path).read() == "0.500000,0.100000,-0.200000,0.800000,1.000000,0.000000,0.000000,0.000000"
+
+
+def test_clusters_from_label_map():
+    labels = np.zeros((480, 640), np.uint16)
+    depth = np.full((480, 640), 1000, np.uint16)
+    labels[100:120, 200:230] = 2  # cluster index 1
+    depth[110, 210] = 0
+    cam = qpso.Camera(525.0, 525.0, 319.5, 239.5)
+    rect = (C.c_int32 * 4)()
+    cen = (C.c_float * 3)()
+    npts = C.c_uint32()
+    out = np.zeros((480, 640), np.uint16)
+    L = qpso.lib()
+    assert L.qpso_clusters_from_labels(labels.ctypes.data, depth.ctypes.data, 1, C.byref(cam), rect, cen,
+                                       C.byref(npts), out.ctypes.data) == 0
+    assert list(rect) == [200, 100, 230, 120] and npts.value == 600
+    assert out.sum() == 599 * 1000 and abs(cen[2] - 1000) < 1e-3
+    assert L.qpso_clusters_from_labels(labels.ctypes.data, depth.ctypes.data, 0, C.byref(cam), rect, cen,
+                                       C.byref(npts), out.ctypes.data) == qpso.E_NOTFOUND
