@@ -27,6 +27,12 @@ namespace {
 
 constexpr int kRows = 480;
 constexpr int kCols = 640;
+// A face whose clipped rectangle has more pixels than this is rasterized by a
+// whole wave, so one large face does not hold its wave's other 63 lanes idle.
+#ifndef QPSO_BIG_FACE
+#define QPSO_BIG_FACE 96
+#endif
+constexpr int kBigFace = QPSO_BIG_FACE;
 
 __device__ __forceinline__ int wave_min(int v) {
 #pragma unroll
@@ -122,6 +128,9 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     __shared__ int32_t s_stale[4];  // carried pixels inside the union box: x0, x1, y0, y1 (inclusive)
     __shared__ int32_t s_degen;
     __shared__ int32_t s_nstale;
+    __shared__ int32_t s_nq;        // small non-flat faces queued
+    __shared__ int32_t s_nqb;       // big faces queued
+    __shared__ int32_t s_take;      // next big face to hand to a wave
     __shared__ uint32_t s_cnt[3];   // seg, tr, n_rendered
     __shared__ unsigned long long s_px[2];
 
@@ -145,7 +154,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         s_box[2] = QPSO_AABB_MIN_SENT; s_box[3] = QPSO_AABB_MAX_SENT;
         s_stale[0] = INT32_MAX; s_stale[1] = INT32_MIN;
         s_stale[2] = INT32_MAX; s_stale[3] = INT32_MIN;
-        s_degen = 0; s_nstale = 0;
+        s_degen = 0; s_nstale = 0; s_nq = 0; s_nqb = 0; s_take = 0;
         s_cnt[0] = s_cnt[1] = s_cnt[2] = 0;
         s_px[0] = s_px[1] = 0;
     }
@@ -158,8 +167,17 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     int32_t* __restrict__ vX = lds;
     int32_t* __restrict__ vY = lds + Ve;
     float* __restrict__ vZ = reinterpret_cast<float*>(lds + 2 * Ve);
-    int32_t* __restrict__ tile = cached ? lds + 3 * Ve : lds;
-    const int32_t tcap = cached ? CAP - 3 * Ve : CAP;
+    constexpr int kQCap = CAP / 32;   // small faces needing the per-pixel depth
+    // faces of more than kBigFace pixels: clipped rectangle (x0 | x1 << 16,
+    // y0 | y1 << 16), constant depth, face entry << 1 | flat
+    constexpr int kBQCap = CAP / 128;
+    int32_t* __restrict__ queue = cached ? lds + 3 * Ve : lds;
+    int32_t* __restrict__ bq_x = queue + kQCap;
+    int32_t* __restrict__ bq_y = bq_x + kBQCap;
+    int32_t* __restrict__ bq_k = bq_y + kBQCap;
+    int32_t* __restrict__ bq_i = bq_k + kBQCap;
+    int32_t* __restrict__ tile = bq_i + kBQCap;
+    const int32_t tcap = CAP - kQCap - 4 * kBQCap - (cached ? 3 * Ve : 0);
 
     // ---- pass A: per-face bbox and AABB over every face (main.h:1340-1353,
     //      AABBreduction main.h:1190-1304, final reduction main.cu:423-546).
@@ -271,7 +289,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     ty0 = max(ty0, 0); ty1 = min(ty1, kRows);
     const bool has_tile = tx0 < tx1 && ty0 < ty1;
     const int32_t TW = has_tile ? tx1 - tx0 : 1;
-    const int32_t BH = tcap / TW;  // >= 1: TW <= 640 and tcap >= CAP / 2
+    const int32_t BH = tcap / TW;  // >= 1: TW <= 640 and tcap >= 7 CAP / 16 >= 4424
 
     float acc_e[SPT];
 #pragma unroll
@@ -285,6 +303,91 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         else atomicOr(S.flags, QPSO_FLAG_STALE_OVERFLOW);
     };
     auto in_tile = [&](int32_t x, int32_t y) { return x >= tx0 && x < tx1 && y >= ty0 && y < ty1; };
+    // the face's 28.4 vertices: from the LDS vertex cache, or projected again
+    auto load_face = [&](const QpsoFaceRef& e, Face& fc) {
+        if (cached) {
+            fc.X[0] = vX[e.v0]; fc.Y[0] = vY[e.v0]; fc.Z[0] = vZ[e.v0];
+            fc.X[1] = vX[e.v1]; fc.Y[1] = vY[e.v1]; fc.Z[1] = vZ[e.v1];
+            fc.X[2] = vX[e.v2]; fc.Y[2] = vY[e.v2]; fc.Z[2] = vZ[e.v2];
+            face_bbox(fc);
+        } else {
+            face_project(mesh, F, e.face, pose, cam, fc);
+        }
+    };
+    // per-pixel depth over [x0, x1) x [y0, y1): edge values at (x0, y0), per
+    // pixel CX -= FDY, from a row's last pixel to the next row's first
+    // CX += FDX + (w - 1) FDY
+    auto raster_exact = [&](const Face& fc, const Setup& su, int32_t x0, int32_t x1, int32_t y0, int32_t y1,
+                            int32_t b0) {
+        if (x0 >= x1 || y0 >= y1) return;
+        const int32_t w = x1 - x0, n = w * (y1 - y0);
+        int32_t CX2 = wsub(wadd(su.CY2, wmul(y0 - fc.miny, su.FDX23)), wmul(x0 - fc.minx, su.FDY23));
+        int32_t CX3 = wsub(wadd(su.CY3, wmul(y0 - fc.miny, su.FDX31)), wmul(x0 - fc.minx, su.FDY31));
+        const int32_t step2 = wsub(0, su.FDY23), step3 = wsub(0, su.FDY31);
+        const int32_t wrap2 = wadd(su.FDX23, wmul(w - 1, su.FDY23));
+        const int32_t wrap3 = wadd(su.FDX31, wmul(w - 1, su.FDY31));
+        const int32_t awrap = TW - w + 1;
+        int32_t* addr = tile + (y0 - b0) * TW + (x0 - tx0);
+        int32_t c = w;
+        for (int32_t m = 0; m < n; ++m) {
+            atomicMin(addr, pixel_depth(CX2, CX3, su));
+            const bool wrap = --c == 0;
+            CX2 = wadd(CX2, wrap ? wrap2 : step2);
+            CX3 = wadd(CX3, wrap ? wrap3 : step3);
+            addr += wrap ? awrap : 1;
+            c = wrap ? w : c;
+        }
+    };
+    // a queued big face walked by the 64 lanes of a wave: lane l starts at
+    // pixel l of the rectangle in row-major order and every step moves 64
+    // pixels on, i.e. sy = 64 / w rows and sx = 64 % w columns, with one wrap
+    // at most.  floor(a / w) for a <= 64, w <= 640 is trunc((a + 0.5) * rcp(w)):
+    // (a + 0.5) / w lies at least 1 / 1280 from an integer, far beyond the
+    // error of the approximate reciprocal.
+    auto raster_wave = [&](int32_t q, int32_t b0) {
+        const int32_t x0 = bq_x[q] & 0xffff, x1 = bq_x[q] >> 16;
+        const int32_t y0 = bq_y[q] & 0xffff, y1 = bq_y[q] >> 16;
+        const int32_t kd = bq_k[q], fi = bq_i[q];
+        const int32_t w = x1 - x0, n = w * (y1 - y0);
+        const int32_t lane = tid & 63;
+        const float rw = __builtin_amdgcn_rcpf((float)w);
+        const int32_t sy = (int32_t)(64.5f * rw), sx = 64 - sy * w;
+        int32_t dy = (int32_t)(((float)lane + 0.5f) * rw), dx = lane - dy * w;
+        const int32_t astep = sy * TW + sx, afix = TW - w;
+        int32_t off = (y0 + dy - b0) * TW + (x0 + dx - tx0);
+        if (fi & 1) {  // flat: one depth over the rectangle
+            for (int32_t m = lane; m < n; m += 64) {
+                atomicMin(tile + off, kd);
+                dx += sx;
+                off += astep;
+                if (dx >= w) { dx -= w; off += afix; }
+            }
+            return;
+        }
+        Face fc;
+        Setup su;
+        load_face(S.faces[fi >> 1], fc);
+        face_setup(fc, su);
+        int32_t CX2 = wsub(wadd(su.CY2, wmul(y0 + dy - fc.miny, su.FDX23)), wmul(x0 + dx - fc.minx, su.FDY23));
+        int32_t CX3 = wsub(wadd(su.CY3, wmul(y0 + dy - fc.miny, su.FDX31)), wmul(x0 + dx - fc.minx, su.FDY31));
+        const int32_t step2 = wsub(wmul(sy, su.FDX23), wmul(sx, su.FDY23));
+        const int32_t step3 = wsub(wmul(sy, su.FDX31), wmul(sx, su.FDY31));
+        const int32_t fix2 = wadd(su.FDX23, wmul(w, su.FDY23));  // x -= w, y += 1
+        const int32_t fix3 = wadd(su.FDX31, wmul(w, su.FDY31));
+        for (int32_t m = lane; m < n; m += 64) {
+            atomicMin(tile + off, pixel_depth(CX2, CX3, su));
+            dx += sx;
+            CX2 = wadd(CX2, step2);
+            CX3 = wadd(CX3, step3);
+            off += astep;
+            if (dx >= w) {
+                dx -= w;
+                CX2 = wadd(CX2, fix2);
+                CX3 = wadd(CX3, fix3);
+                off += afix;
+            }
+        }
+    };
 
     // ---- cluster pixels outside the tile region: depth buffer reads 60000
     //      there, so only "observed but not rendered" counts (order-free)
@@ -315,7 +418,10 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
 
         // ---- raster: bbox fill with ds_min_i32 (main.h:1355-1488).  Each lane
         //      walks its face's clipped bbox as one flat pixel loop (a nested
-        //      row/column loop would diverge as max rows x max columns).
+        //      row/column loop would diverge as max rows x max columns).  Faces
+        //      whose written depth is provably one integer over the rectangle
+        //      (flat_depth) store that constant; the others are queued and
+        //      rasterized with the per-pixel depth after the pass.
         int k = 0;
         QpsoFaceRef nxt = tid < S.n_faces ? S.faces[tid] : QpsoFaceRef{0, 0, 0, INT32_MAX};
         for (int i = tid; i < S.n_faces; i += BT, ++k) {
@@ -325,15 +431,8 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             bool later = false;
             if (e.face < Fe) {
                 Face fc;
-                if (cached) {
-                    fc.X[0] = vX[e.v0]; fc.Y[0] = vY[e.v0]; fc.Z[0] = vZ[e.v0];
-                    fc.X[1] = vX[e.v1]; fc.Y[1] = vY[e.v1]; fc.Z[1] = vZ[e.v1];
-                    fc.X[2] = vX[e.v2]; fc.Y[2] = vY[e.v2]; fc.Z[2] = vZ[e.v2];
-                    face_bbox(fc);
-                } else {
-                    face_project(mesh, F, e.face, pose, cam, fc);
-                }
                 Setup su;
+                load_face(e, fc);
                 const int32_t x0 = max(fc.minx, tx0), x1 = min(fc.maxx, tx1);
                 if (x0 < x1 && fc.miny < ty1 && fc.maxy > b0 && face_setup(fc, su)) {
                     later = fc.maxy > b1;
@@ -341,23 +440,28 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
                     if (y0 < y1) {
                         const int32_t w = x1 - x0, n = w * (y1 - y0);
                         px_r += (unsigned long long)n;
-                        // edge values at (x0, y0); per pixel CX -= FDY; from the last
-                        // pixel of a row to the first of the next, CX += FDX + (w-1) FDY
-                        int32_t CX2 = wsub(wadd(su.CY2, wmul(y0 - fc.miny, su.FDX23)), wmul(x0 - fc.minx, su.FDY23));
-                        int32_t CX3 = wsub(wadd(su.CY3, wmul(y0 - fc.miny, su.FDX31)), wmul(x0 - fc.minx, su.FDY31));
-                        const int32_t step2 = wsub(0, su.FDY23), step3 = wsub(0, su.FDY31);
-                        const int32_t wrap2 = wadd(su.FDX23, wmul(w - 1, su.FDY23));
-                        const int32_t wrap3 = wadd(su.FDX31, wmul(w - 1, su.FDY31));
-                        const int32_t awrap = TW - w + 1;
-                        int32_t* addr = tile + (y0 - b0) * TW + (x0 - tx0);
-                        int32_t c = w;
-                        for (int32_t m = 0; m < n; ++m) {
-                            atomicMin(addr, pixel_depth(CX2, CX3, su));
-                            const bool wrap = --c == 0;
-                            CX2 = wadd(CX2, wrap ? wrap2 : step2);
-                            CX3 = wadd(CX3, wrap ? wrap3 : step3);
-                            addr += wrap ? awrap : 1;
-                            c = wrap ? w : c;
+                        int32_t kd = 0, qb;
+                        const bool flat = flat_depth(fc, su, x0, x1, y0, y1, kd);
+                        if (n > kBigFace && (qb = atomicAdd(&s_nqb, 1)) < kBQCap) {
+                            // rasterized by a whole wave after the pass
+                            bq_x[qb] = x0 | (x1 << 16);
+                            bq_y[qb] = y0 | (y1 << 16);
+                            bq_k[qb] = kd;
+                            bq_i[qb] = (i << 1) | (flat ? 1 : 0);
+                        } else if (flat) {
+                            const int32_t awrap = TW - w + 1;
+                            int32_t* addr = tile + (y0 - b0) * TW + (x0 - tx0);
+                            int32_t c = w;
+                            for (int32_t m = 0; m < n; ++m) {
+                                atomicMin(addr, kd);
+                                const bool wrap = --c == 0;
+                                addr += wrap ? awrap : 1;
+                                c = wrap ? w : c;
+                            }
+                        } else {
+                            const int q = atomicAdd(&s_nq, 1);
+                            if (q < kQCap) queue[q] = i;
+                            else raster_exact(fc, su, x0, x1, y0, y1, b0);
                         }
                     }
                 }
@@ -365,6 +469,26 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             if (use_mask && !later) fmask &= ~(1ull << k);
         }
         __syncthreads();
+        // queued small faces: per-pixel depth, one face per thread
+        const int nq = min(s_nq, kQCap);
+        for (int q = tid; q < nq; q += BT) {
+            Face fc;
+            Setup su;
+            load_face(S.faces[queue[q]], fc);
+            face_setup(fc, su);
+            raster_exact(fc, su, max(fc.minx, tx0), min(fc.maxx, tx1), max(fc.miny, b0), min(fc.maxy, b1), b0);
+        }
+        // queued big faces: one face per wave, taken in turn
+        const int nqb = min(s_nqb, kBQCap);
+        for (;;) {
+            int q = 0;
+            if ((tid & 63) == 0) q = atomicAdd(&s_take, 1);
+            q = __shfl(q, 0);
+            if (q >= nqb) break;
+            raster_wave(q, b0);
+        }
+        __syncthreads();
+        if (tid == 0) { s_nq = 0; s_nqb = 0; s_take = 0; }
 
         if (S.dbg_frames) {
             int32_t* fr = S.dbg_frames + (size_t)p * kRows * kCols;

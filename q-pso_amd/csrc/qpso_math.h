@@ -296,6 +296,45 @@ QD int32_t pixel_depth(int32_t CX2, int32_t CX3, const Setup& s) {
     return rint_sat(d);
 }
 
+// Exact test for a face whose written depth is one integer k over the pixel
+// rectangle [x0, x1) x [y0, y1) of its bbox.  The edge value at a pixel is
+// CY + FDX (y - miny) - FDY (x - minx) mod 2^32; if that affine function stays
+// inside [INT32_MIN, INT32_MAX - 15] at the four corners it does so everywhere,
+// no wrap occurs, and cx = (CX + 15) >> 8 ranges between its corner values.
+// The depth fma(cx3, z3, fma(cx2, z2, z1)) is monotone in cx2, cx3 (for fixed
+// z2, z3) and in the addend, and rint is monotone, so the corner combinations
+// bound every pixel's written value exactly.
+QD bool flat_depth(const Face& fc, const Setup& s, int32_t x0, int32_t x1, int32_t y0, int32_t y1,
+                   int32_t& k) {
+    const long long ya = y0 - fc.miny, yb = y1 - 1 - fc.miny;
+    const long long xa = x0 - fc.minx, xb = x1 - 1 - fc.minx;
+    int32_t lo2, hi2, lo3, hi3;
+    {
+        const long long e = (long long)s.CY2 + (long long)s.FDX23 * ya - (long long)s.FDY23 * xa;
+        const long long dx = -(long long)s.FDY23 * (xb - xa), dy = (long long)s.FDX23 * (yb - ya);
+        const long long mn = e + (dx < 0 ? dx : 0) + (dy < 0 ? dy : 0);
+        const long long mx = e + (dx > 0 ? dx : 0) + (dy > 0 ? dy : 0);
+        if (mn < (long long)INT32_MIN || mx > (long long)INT32_MAX - 15) return false;
+        lo2 = (int32_t)((mn + 15) >> 8);
+        hi2 = (int32_t)((mx + 15) >> 8);
+    }
+    {
+        const long long e = (long long)s.CY3 + (long long)s.FDX31 * ya - (long long)s.FDY31 * xa;
+        const long long dx = -(long long)s.FDY31 * (xb - xa), dy = (long long)s.FDX31 * (yb - ya);
+        const long long mn = e + (dx < 0 ? dx : 0) + (dy < 0 ? dy : 0);
+        const long long mx = e + (dx > 0 ? dx : 0) + (dy > 0 ? dy : 0);
+        if (mn < (long long)INT32_MIN || mx > (long long)INT32_MAX - 15) return false;
+        lo3 = (int32_t)((mn + 15) >> 8);
+        hi3 = (int32_t)((mx + 15) >> 8);
+    }
+    const float fa = __builtin_fmaf((float)lo2, s.z2, s.z1), fb = __builtin_fmaf((float)hi2, s.z2, s.z1);
+    const float d0 = __builtin_fmaf((float)lo3, s.z3, fa), d1 = __builtin_fmaf((float)lo3, s.z3, fb);
+    const float d2 = __builtin_fmaf((float)hi3, s.z3, fa), d3 = __builtin_fmaf((float)hi3, s.z3, fb);
+    if (d0 != d0 || d1 != d1 || d2 != d2 || d3 != d3) return false;
+    k = rint_sat(__builtin_fminf(__builtin_fminf(d0, d1), __builtin_fminf(d2, d3)));
+    return k == rint_sat(__builtin_fmaxf(__builtin_fmaxf(d0, d1), __builtin_fmaxf(d2, d3)));
+}
+
 // Fitness row index y(t) = trunc(f32(t) * rcp(f32(W))) (main.cu:645).
 QD int32_t row_of(int32_t t, float rW) { return (int32_t)__builtin_truncf((float)t * rW); }
 
