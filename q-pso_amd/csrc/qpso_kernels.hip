@@ -34,6 +34,23 @@ constexpr int kCols = 640;
 #endif
 constexpr int kBigFace = QPSO_BIG_FACE;
 
+// Global-memory pointers: loads through them cannot alias LDS, so the compiler
+// may move them across LDS stores and atomics (generic pointers could).
+#define QG __attribute__((address_space(1)))
+typedef int32_t qv4 __attribute__((ext_vector_type(4)));
+typedef int32_t qv2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ QpsoFaceRef load_ref(const QG QpsoFaceRef* p, int i) {
+    const qv4 v = reinterpret_cast<const QG qv4*>(p)[i];
+    return QpsoFaceRef{v.x, v.y, v.z, v.w};
+}
+__device__ __forceinline__ QpsoStale load_stale(const QG QpsoStale* p, int i) {
+    const qv2 v = reinterpret_cast<const QG qv2*>(p)[i];
+    return QpsoStale{v.x, v.y};
+}
+__device__ __forceinline__ void store_stale(QG QpsoStale* p, int i, int32_t pos, int32_t val) {
+    reinterpret_cast<QG qv2*>(p)[i] = qv2{pos, val};
+}
+
 __device__ __forceinline__ int wave_min(int v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
@@ -47,6 +64,16 @@ __device__ __forceinline__ int wave_max(int v) {
 __device__ __forceinline__ unsigned wave_sum(unsigned v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += (unsigned)__shfl_xor((int)v, o);
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum64(unsigned long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned lo = (unsigned)__shfl_xor((int)(unsigned)v, o);
+        const unsigned hi = (unsigned)__shfl_xor((int)(unsigned)(v >> 32), o);
+        v += ((unsigned long long)hi << 32) | lo;
+    }
     return v;
 }
 
@@ -148,6 +175,8 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     const Cam cam{S.fx, S.fy, S.cx, S.cy};
     const float* __restrict__ mesh = S.mesh;
     const int F = S.F, Fe = S.F_eff;
+    const QG QpsoFaceRef* __restrict__ faces = (const QG QpsoFaceRef*)S.faces;
+    const int n_faces = S.n_faces;
 
     if (tid == 0) {
         s_box[0] = QPSO_AABB_MIN_SENT; s_box[1] = QPSO_AABB_MAX_SENT;
@@ -176,8 +205,11 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     int32_t* __restrict__ bq_y = bq_x + kBQCap;
     int32_t* __restrict__ bq_k = bq_y + kBQCap;
     int32_t* __restrict__ bq_i = bq_k + kBQCap;
-    int32_t* __restrict__ tile = bq_i + kBQCap;
-    const int32_t tcap = CAP - kQCap - 4 * kBQCap - (cached ? 3 * Ve : 0);
+    // per wave: ring of culled face entries waiting to be drawn
+    constexpr int kRing = 128;
+    int32_t* __restrict__ rings = bq_i + kBQCap;
+    int32_t* __restrict__ tile = rings + (BT / 64) * kRing;
+    const int32_t tcap = CAP - kQCap - 4 * kBQCap - 2 * BT - (cached ? 3 * Ve : 0);
 
     // ---- pass A: per-face bbox and AABB over every face (main.h:1340-1353,
     //      AABBreduction main.h:1190-1304, final reduction main.cu:423-546).
@@ -188,9 +220,9 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         int by0 = QPSO_AABB_MIN_SENT, by1 = QPSO_AABB_MAX_SENT;
         int dg = 0;
         if (cached) {
-            const float* __restrict__ wx = S.verts;
-            const float* __restrict__ wy = wx + S.V;
-            const float* __restrict__ wz = wy + S.V;
+            const QG float* __restrict__ wx = (const QG float*)S.verts;
+            const QG float* __restrict__ wy = wx + S.V;
+            const QG float* __restrict__ wz = wy + S.V;
             int32_t xl = INT32_MAX, xh = INT32_MIN, yl = INT32_MAX, yh = INT32_MIN;
             for (int v = tid; v < Ve; v += BT) {
                 int32_t X, Y;
@@ -236,9 +268,9 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     }
     const int par = iter & 1;
     const int cap = S.stale_cap;
-    const QpsoStale* __restrict__ old = S.stale[par] + (size_t)p * cap;
+    const QG QpsoStale* __restrict__ old = (const QG QpsoStale*)S.stale[par] + (size_t)p * cap;
     const int n_old = S.stale_cnt[par][p];
-    QpsoStale* __restrict__ nw = S.stale[par ^ 1] + (size_t)p * cap;
+    QG QpsoStale* __restrict__ nw = (QG QpsoStale*)S.stale[par ^ 1] + (size_t)p * cap;
 
     if (degen) {  // guard G: NaN fitness, buffer reset
         if (tid == 0) {
@@ -255,13 +287,13 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     const Visits vis{W, H, T, rcp((float)W)};
     const int32_t rx0 = S.rect[0], ry0 = S.rect[1], rx1 = S.rect[2], ry1 = S.rect[3];
     const int32_t rw = rx1 - rx0;
-    const uint16_t* __restrict__ obs = S.obs;
+    const QG uint16_t* __restrict__ obs = (const QG uint16_t*)S.obs;
 
     // carried pixels inside the union box join the tile region
     {
         int32_t sx0 = INT32_MAX, sx1 = INT32_MIN, sy0 = INT32_MAX, sy1 = INT32_MIN;
         for (int k = tid; k < n_old; k += BT) {
-            const QpsoStale e = old[k];
+            const QpsoStale e = load_stale(old, k);
             const int32_t ex = e.pos & 0xffff, ey = e.pos >> 16;
             if (ex >= minX && ex < maxX && ey >= minY && ey <= maxY) {
                 sx0 = min(sx0, ex); sx1 = max(sx1, ex);
@@ -289,7 +321,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     ty0 = max(ty0, 0); ty1 = min(ty1, kRows);
     const bool has_tile = tx0 < tx1 && ty0 < ty1;
     const int32_t TW = has_tile ? tx1 - tx0 : 1;
-    const int32_t BH = tcap / TW;  // >= 1: TW <= 640 and tcap >= 7 CAP / 16 >= 4424
+    const int32_t BH = tcap / TW;  // >= 1: TW <= 640 and tcap >= 7 CAP / 16 - 2 BT >= 3912
 
     float acc_e[SPT];
 #pragma unroll
@@ -299,7 +331,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
 
     auto push_stale = [&](int32_t x, int32_t y, int32_t v) {
         const int k = atomicAdd(&s_nstale, 1);
-        if (k < cap) nw[k] = QpsoStale{(y << 16) | x, v};
+        if (k < cap) store_stale(nw, k, (y << 16) | x, v);
         else atomicOr(S.flags, QPSO_FLAG_STALE_OVERFLOW);
     };
     auto in_tile = [&](int32_t x, int32_t y) { return x >= tx0 && x < tx1 && y >= ty0 && y < ty1; };
@@ -366,7 +398,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         }
         Face fc;
         Setup su;
-        load_face(S.faces[fi >> 1], fc);
+        load_face(load_ref(faces, fi >> 1), fc);
         face_setup(fc, su);
         int32_t CX2 = wsub(wadd(su.CY2, wmul(y0 + dy - fc.miny, su.FDX23)), wmul(x0 + dx - fc.minx, su.FDY23));
         int32_t CX3 = wsub(wadd(su.CY3, wmul(y0 + dy - fc.miny, su.FDX31)), wmul(x0 + dx - fc.minx, su.FDY31));
@@ -402,7 +434,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     }
 
     uint64_t fmask = ~0ull;  // bit k: face entry tid + k*BT still reaches later bands
-    const bool use_mask = S.n_faces <= 64 * BT;
+    const bool use_mask = n_faces <= 64 * BT;
     for (int32_t b0 = ty0; has_tile && b0 < ty1; b0 += BH) {
         const int32_t b1 = min(b0 + BH, ty1);  // band rows [b0, b1)
         const int32_t nt = (b1 - b0) * TW;
@@ -410,71 +442,97 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         __syncthreads();
         // pixels carried over from earlier iterations
         for (int k = tid; k < n_old; k += BT) {
-            const QpsoStale e = old[k];
+            const QpsoStale e = load_stale(old, k);
             const int32_t ex = e.pos & 0xffff, ey = e.pos >> 16;
             if (ey >= b0 && ey < b1 && ex >= tx0 && ex < tx1) tile[(ey - b0) * TW + ex - tx0] = e.val;
         }
         __syncthreads();
 
-        // ---- raster: bbox fill with ds_min_i32 (main.h:1355-1488).  Each lane
-        //      walks its face's clipped bbox as one flat pixel loop (a nested
-        //      row/column loop would diverge as max rows x max columns).  Faces
-        //      whose written depth is provably one integer over the rectangle
-        //      (flat_depth) store that constant; the others are queued and
-        //      rasterized with the per-pixel depth after the pass.
+        // ---- raster: bbox fill with ds_min_i32 (main.h:1355-1488) in two steps
+        //      per wave.  Cull: the wave scans 64 face entries at a time and
+        //      appends those that are front-facing and reach the band and the
+        //      tile to its ring in LDS.  Draw: whenever 64 are waiting, one per
+        //      lane, each lane walks its face's clipped bbox as one flat pixel
+        //      loop (a nested row/column loop would diverge as max rows x max
+        //      columns).  Faces whose written depth is provably one integer
+        //      over the rectangle (flat_depth) store that constant; the others
+        //      are queued and rasterized with the per-pixel depth after the
+        //      pass; faces of more than kBigFace pixels go to the wave queue.
+        auto draw = [&](int32_t i) {
+            Face fc;
+            Setup su;
+            load_face(load_ref(faces, i), fc);
+            face_setup(fc, su);
+            const int32_t x0 = max(fc.minx, tx0), x1 = min(fc.maxx, tx1);
+            const int32_t y0 = max(fc.miny, b0), y1 = min(fc.maxy, b1);
+            const int32_t w = x1 - x0, n = w * (y1 - y0);
+            px_r += (unsigned long long)n;
+            int32_t kd = 0, qb;
+            const bool flat = flat_depth(fc, su, x0, x1, y0, y1, kd);
+            if (n > kBigFace && (qb = atomicAdd(&s_nqb, 1)) < kBQCap) {
+                bq_x[qb] = x0 | (x1 << 16);
+                bq_y[qb] = y0 | (y1 << 16);
+                bq_k[qb] = kd;
+                bq_i[qb] = (i << 1) | (flat ? 1 : 0);
+            } else if (flat) {
+                const int32_t awrap = TW - w + 1;
+                int32_t* addr = tile + (y0 - b0) * TW + (x0 - tx0);
+                int32_t c = w;
+                for (int32_t m = 0; m < n; ++m) {
+                    atomicMin(addr, kd);
+                    const bool wrap = --c == 0;
+                    addr += wrap ? awrap : 1;
+                    c = wrap ? w : c;
+                }
+            } else {
+                const int q = atomicAdd(&s_nq, 1);
+                if (q < kQCap) queue[q] = i;
+                else raster_exact(fc, su, x0, x1, y0, y1, b0);
+            }
+        };
+        int32_t* __restrict__ ring = rings + (tid >> 6) * kRing;
+        const int lane = tid & 63;
+        const uint64_t below = (1ull << lane) - 1ull;
+        int head = 0, cnt = 0;  // wave-uniform ring state
         int k = 0;
-        QpsoFaceRef nxt = tid < S.n_faces ? S.faces[tid] : QpsoFaceRef{0, 0, 0, INT32_MAX};
-        for (int i = tid; i < S.n_faces; i += BT, ++k) {
+        QpsoFaceRef nxt{0, 0, 0, INT32_MAX};
+        if (tid < n_faces) nxt = load_ref(faces, tid);
+        for (int base = tid - lane; base < n_faces; base += BT, ++k) {
+            const int i = base + lane;  // = tid + k * BT
             const QpsoFaceRef e = nxt;  // prefetched one iteration ahead
-            if (i + BT < S.n_faces) nxt = S.faces[i + BT];
-            if (use_mask && !((fmask >> k) & 1ull)) continue;
-            bool later = false;
-            if (e.face < Fe) {
-                Face fc;
-                Setup su;
-                load_face(e, fc);
-                const int32_t x0 = max(fc.minx, tx0), x1 = min(fc.maxx, tx1);
-                if (x0 < x1 && fc.miny < ty1 && fc.maxy > b0 && face_setup(fc, su)) {
-                    later = fc.maxy > b1;
-                    const int32_t y0 = max(fc.miny, b0), y1 = min(fc.maxy, b1);
-                    if (y0 < y1) {
-                        const int32_t w = x1 - x0, n = w * (y1 - y0);
-                        px_r += (unsigned long long)n;
-                        int32_t kd = 0, qb;
-                        const bool flat = flat_depth(fc, su, x0, x1, y0, y1, kd);
-                        if (n > kBigFace && (qb = atomicAdd(&s_nqb, 1)) < kBQCap) {
-                            // rasterized by a whole wave after the pass
-                            bq_x[qb] = x0 | (x1 << 16);
-                            bq_y[qb] = y0 | (y1 << 16);
-                            bq_k[qb] = kd;
-                            bq_i[qb] = (i << 1) | (flat ? 1 : 0);
-                        } else if (flat) {
-                            const int32_t awrap = TW - w + 1;
-                            int32_t* addr = tile + (y0 - b0) * TW + (x0 - tx0);
-                            int32_t c = w;
-                            for (int32_t m = 0; m < n; ++m) {
-                                atomicMin(addr, kd);
-                                const bool wrap = --c == 0;
-                                addr += wrap ? awrap : 1;
-                                c = wrap ? w : c;
-                            }
-                        } else {
-                            const int q = atomicAdd(&s_nq, 1);
-                            if (q < kQCap) queue[q] = i;
-                            else raster_exact(fc, su, x0, x1, y0, y1, b0);
-                        }
+            if (i + BT < n_faces) nxt = load_ref(faces, i + BT);
+            bool keep = false;
+            if (i < n_faces && (!use_mask || ((fmask >> k) & 1ull))) {
+                bool later = false;
+                if (e.face < Fe) {
+                    Face fc;
+                    load_face(e, fc);
+                    if (max(fc.minx, tx0) < min(fc.maxx, tx1) && fc.miny < ty1 && fc.maxy > b0 && face_front(fc)) {
+                        later = fc.maxy > b1;
+                        keep = fc.miny < b1;
                     }
                 }
+                if (use_mask && !later) fmask &= ~(1ull << k);
             }
-            if (use_mask && !later) fmask &= ~(1ull << k);
+            const uint64_t m = __ballot(keep);
+            if (keep) ring[(head + cnt + __popcll(m & below)) & (kRing - 1)] = i;
+            cnt += __popcll(m);
+            __builtin_amdgcn_wave_barrier();
+            if (cnt >= 64) {
+                draw(ring[(head + lane) & (kRing - 1)]);
+                head = (head + 64) & (kRing - 1);
+                cnt -= 64;
+                __builtin_amdgcn_wave_barrier();
+            }
         }
+        if (lane < cnt) draw(ring[(head + lane) & (kRing - 1)]);
         __syncthreads();
         // queued small faces: per-pixel depth, one face per thread
         const int nq = min(s_nq, kQCap);
         for (int q = tid; q < nq; q += BT) {
             Face fc;
             Setup su;
-            load_face(S.faces[queue[q]], fc);
+            load_face(load_ref(faces, queue[q]), fc);
             face_setup(fc, su);
             raster_exact(fc, su, max(fc.minx, tx0), min(fc.maxx, tx1), max(fc.miny, b0), min(fc.maxy, b1), b0);
         }
@@ -566,7 +624,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
 
     // carried pixels outside this iteration's union box stay as they are
     for (int k = tid; k < n_old; k += BT) {
-        const QpsoStale e = old[k];
+        const QpsoStale e = load_stale(old, k);
         const int32_t ex = e.pos & 0xffff, ey = e.pos >> 16;
         if (!(ex >= minX && ex < maxX && ey >= minY && ey <= maxY)) push_stale(ex, ey, e.val);
     }
@@ -598,8 +656,12 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     if ((tid & 63) == 0) {
         atomicAdd(&s_cnt[0], cs); atomicAdd(&s_cnt[1], ct); atomicAdd(&s_cnt[2], cr);
     }
-    atomicAdd(&s_px[0], px_r);
-    atomicAdd(&s_px[1], px_f);
+    px_r = wave_sum64(px_r);
+    px_f = wave_sum64(px_f);
+    if ((tid & 63) == 0) {
+        atomicAdd(&s_px[0], px_r);
+        atomicAdd(&s_px[1], px_f);
+    }
 
     float* red = reinterpret_cast<float*>(tile);  // the tile is dead now
     red[tid] = acc_e[0];

@@ -263,6 +263,14 @@ struct Setup {
     float z1, z2, z3;
 };
 
+// The faces face_setup accepts: positive signed area (counter-clockwise in
+// 28.4 screen coordinates) and a non-empty bbox.
+QD bool face_front(const Face& fc) {
+    const int32_t DX12 = wsub(fc.X[0], fc.X[1]), DX31 = wsub(fc.X[2], fc.X[0]);
+    const int32_t DY12 = wsub(fc.Y[0], fc.Y[1]), DY31 = wsub(fc.Y[2], fc.Y[0]);
+    return wsub(wmul(DX12, DY31), wmul(DX31, DY12)) > 0 && fc.maxx > fc.minx && fc.maxy > fc.miny;
+}
+
 QD bool face_setup(const Face& fc, Setup& s) {
     const int32_t X1 = fc.X[0], X2 = fc.X[1], X3 = fc.X[2];
     const int32_t Y1 = fc.Y[0], Y2 = fc.Y[1], Y3 = fc.Y[2];
@@ -296,6 +304,32 @@ QD int32_t pixel_depth(int32_t CX2, int32_t CX3, const Setup& s) {
     return rint_sat(d);
 }
 
+// Range [lo, hi] of cx = (CX + 15) >> 8 over the offsets [ia, ib] x [ja, jb]
+// of an edge CX = CY + FDX j - FDY i, in 64 bits; false if a corner leaves
+// [INT32_MIN, INT32_MAX - 15], where the 32-bit value would wrap.
+QD bool edge_range64(int32_t CY, int32_t FDX, int32_t FDY, int32_t ia, int32_t ib, int32_t ja, int32_t jb,
+                     int32_t& lo, int32_t& hi) {
+    const long long e = (long long)CY + (long long)FDX * ja - (long long)FDY * ia;
+    const long long dx = -(long long)FDY * (ib - ia), dy = (long long)FDX * (jb - ja);
+    const long long mn = e + (dx < 0 ? dx : 0) + (dy < 0 ? dy : 0);
+    const long long mx = e + (dx > 0 ? dx : 0) + (dy > 0 ? dy : 0);
+    if (mn < (long long)INT32_MIN || mx > (long long)INT32_MAX - 15) return false;
+    lo = (int32_t)((mn + 15) >> 8);
+    hi = (int32_t)((mx + 15) >> 8);
+    return true;
+}
+
+// The same in 32 bits, for |FDX|, |FDY| < 2^18, |CY| < 2^28 and offsets below
+// 1024: every product is below 2^28 in magnitude and every corner value below
+// 5 * 2^28, so nothing wraps.
+QD void edge_range32(int32_t CY, int32_t FDX, int32_t FDY, int32_t ia, int32_t ib, int32_t ja, int32_t jb,
+                     int32_t& lo, int32_t& hi) {
+    const int32_t e = CY + FDX * ja - FDY * ia;
+    const int32_t dx = -FDY * (ib - ia), dy = FDX * (jb - ja);
+    lo = (e + min(dx, 0) + min(dy, 0) + 15) >> 8;
+    hi = (e + max(dx, 0) + max(dy, 0) + 15) >> 8;
+}
+
 // Exact test for a face whose written depth is one integer k over the pixel
 // rectangle [x0, x1) x [y0, y1) of its bbox.  The edge value at a pixel is
 // CY + FDX (y - miny) - FDY (x - minx) mod 2^32; if that affine function stays
@@ -303,29 +337,21 @@ QD int32_t pixel_depth(int32_t CX2, int32_t CX3, const Setup& s) {
 // no wrap occurs, and cx = (CX + 15) >> 8 ranges between its corner values.
 // The depth fma(cx3, z3, fma(cx2, z2, z1)) is monotone in cx2, cx3 (for fixed
 // z2, z3) and in the addend, and rint is monotone, so the corner combinations
-// bound every pixel's written value exactly.
+// bound every pixel's written value exactly.  The rectangle lies inside the
+// bbox, so the offsets are >= 0.
 QD bool flat_depth(const Face& fc, const Setup& s, int32_t x0, int32_t x1, int32_t y0, int32_t y1,
                    int32_t& k) {
-    const long long ya = y0 - fc.miny, yb = y1 - 1 - fc.miny;
-    const long long xa = x0 - fc.minx, xb = x1 - 1 - fc.minx;
+    const int32_t ia = x0 - fc.minx, ib = x1 - 1 - fc.minx, ja = y0 - fc.miny, jb = y1 - 1 - fc.miny;
+    const uint32_t steps = (uint32_t)__builtin_abs(s.FDX23) | (uint32_t)__builtin_abs(s.FDY23) |
+                           (uint32_t)__builtin_abs(s.FDX31) | (uint32_t)__builtin_abs(s.FDY31);
+    const uint32_t origin = (uint32_t)__builtin_abs(s.CY2) | (uint32_t)__builtin_abs(s.CY3);
     int32_t lo2, hi2, lo3, hi3;
-    {
-        const long long e = (long long)s.CY2 + (long long)s.FDX23 * ya - (long long)s.FDY23 * xa;
-        const long long dx = -(long long)s.FDY23 * (xb - xa), dy = (long long)s.FDX23 * (yb - ya);
-        const long long mn = e + (dx < 0 ? dx : 0) + (dy < 0 ? dy : 0);
-        const long long mx = e + (dx > 0 ? dx : 0) + (dy > 0 ? dy : 0);
-        if (mn < (long long)INT32_MIN || mx > (long long)INT32_MAX - 15) return false;
-        lo2 = (int32_t)((mn + 15) >> 8);
-        hi2 = (int32_t)((mx + 15) >> 8);
-    }
-    {
-        const long long e = (long long)s.CY3 + (long long)s.FDX31 * ya - (long long)s.FDY31 * xa;
-        const long long dx = -(long long)s.FDY31 * (xb - xa), dy = (long long)s.FDX31 * (yb - ya);
-        const long long mn = e + (dx < 0 ? dx : 0) + (dy < 0 ? dy : 0);
-        const long long mx = e + (dx > 0 ? dx : 0) + (dy > 0 ? dy : 0);
-        if (mn < (long long)INT32_MIN || mx > (long long)INT32_MAX - 15) return false;
-        lo3 = (int32_t)((mn + 15) >> 8);
-        hi3 = (int32_t)((mx + 15) >> 8);
+    if (steps < (1u << 18) && origin < (1u << 28) && (uint32_t)(ib | jb) < 1024u) {
+        edge_range32(s.CY2, s.FDX23, s.FDY23, ia, ib, ja, jb, lo2, hi2);
+        edge_range32(s.CY3, s.FDX31, s.FDY31, ia, ib, ja, jb, lo3, hi3);
+    } else if (!edge_range64(s.CY2, s.FDX23, s.FDY23, ia, ib, ja, jb, lo2, hi2) ||
+               !edge_range64(s.CY3, s.FDX31, s.FDY31, ia, ib, ja, jb, lo3, hi3)) {
+        return false;
     }
     const float fa = __builtin_fmaf((float)lo2, s.z2, s.z1), fb = __builtin_fmaf((float)hi2, s.z2, s.z1);
     const float d0 = __builtin_fmaf((float)lo3, s.z3, fa), d1 = __builtin_fmaf((float)lo3, s.z3, fb);
