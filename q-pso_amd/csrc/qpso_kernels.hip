@@ -208,8 +208,10 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     // per wave: ring of culled face entries waiting to be drawn
     constexpr int kRing = 128;
     int32_t* __restrict__ rings = bq_i + kBQCap;
-    int32_t* __restrict__ tile = rings + (BT / 64) * kRing;
-    const int32_t tcap = CAP - kQCap - 4 * kBQCap - 2 * BT - (cached ? 3 * Ve : 0);
+    // the tile starts on a 16-byte boundary for 128-bit fills
+    const int32_t tile_off = (int32_t)(rings - lds) + (BT / 64) * kRing;
+    int32_t* __restrict__ tile = lds + ((tile_off + 3) & ~3);
+    const int32_t tcap = CAP - ((tile_off + 3) & ~3);
 
     // ---- pass A: per-face bbox and AABB over every face (main.h:1340-1353,
     //      AABBreduction main.h:1190-1304, final reduction main.cu:423-546).
@@ -321,7 +323,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     ty0 = max(ty0, 0); ty1 = min(ty1, kRows);
     const bool has_tile = tx0 < tx1 && ty0 < ty1;
     const int32_t TW = has_tile ? tx1 - tx0 : 1;
-    const int32_t BH = tcap / TW;  // >= 1: TW <= 640 and tcap >= 7 CAP / 16 - 2 BT >= 3912
+    const int32_t BH = tcap / TW;  // >= 1: TW <= 640 and tcap >= 7 CAP / 16 - 2 BT - 3 >= 3909
 
     float acc_e[SPT];
 #pragma unroll
@@ -334,7 +336,6 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         if (k < cap) store_stale(nw, k, (y << 16) | x, v);
         else atomicOr(S.flags, QPSO_FLAG_STALE_OVERFLOW);
     };
-    auto in_tile = [&](int32_t x, int32_t y) { return x >= tx0 && x < tx1 && y >= ty0 && y < ty1; };
     // the face's 28.4 vertices: from the LDS vertex cache, or projected again
     auto load_face = [&](const QpsoFaceRef& e, Face& fc) {
         if (cached) {
@@ -422,14 +423,29 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     };
 
     // ---- cluster pixels outside the tile region: depth buffer reads 60000
-    //      there, so only "observed but not rendered" counts (order-free)
+    //      there, so only "observed but not rendered" counts (order-free).
+    //      The rect minus the tile is at most four strips: rows above and
+    //      below the tile, and left and right of it within its rows.
     {
-        const int32_t dq = BT / rw, dr = BT % rw;
-        int32_t y = ry0 + tid / rw, x = rx0 + tid % rw;
-        for (int i = tid; i < rw * (ry1 - ry0); i += BT) {
-            if (!in_tile(x, y) && obs[i] != 0) n_seg += (uint32_t)vis.at(x - minX, y - minY);
-            x += dr; y += dq;
-            if (x >= rx1) { x -= rw; ++y; }
+        auto strip = [&](int32_t x0, int32_t x1, int32_t y0, int32_t y1) {
+            if (x0 >= x1 || y0 >= y1) return;
+            const int32_t sw = x1 - x0, n = sw * (y1 - y0);
+            const int32_t dq = BT / sw, dr = BT % sw;
+            int32_t y = y0 + tid / sw, x = x0 + tid % sw;
+            for (int i = tid; i < n; i += BT) {
+                if (obs[(y - ry0) * rw + (x - rx0)] != 0) n_seg += (uint32_t)vis.at(x - minX, y - minY);
+                x += dr; y += dq;
+                if (x >= x1) { x -= sw; ++y; }
+            }
+        };
+        if (has_tile) {
+            const int32_t my0 = max(ry0, ty0), my1 = min(ry1, ty1);
+            strip(rx0, rx1, ry0, min(ry1, ty0));
+            strip(rx0, rx1, max(ry0, ty1), ry1);
+            strip(rx0, min(rx1, tx0), my0, my1);
+            strip(max(rx0, tx1), rx1, my0, my1);
+        } else {
+            strip(rx0, rx1, ry0, ry1);
         }
     }
 
@@ -438,7 +454,13 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     for (int32_t b0 = ty0; has_tile && b0 < ty1; b0 += BH) {
         const int32_t b1 = min(b0 + BH, ty1);  // band rows [b0, b1)
         const int32_t nt = (b1 - b0) * TW;
-        for (int i = tid; i < nt; i += BT) tile[i] = QPSO_EMPTY;
+        {
+            typedef int32_t v4i __attribute__((ext_vector_type(4)));
+            v4i* t4 = reinterpret_cast<v4i*>(tile);
+            const v4i e4 = {QPSO_EMPTY, QPSO_EMPTY, QPSO_EMPTY, QPSO_EMPTY};
+            for (int i = tid; i < nt / 4; i += BT) t4[i] = e4;
+            if (tid < (nt & 3)) tile[(nt & ~3) + tid] = QPSO_EMPTY;
+        }
         __syncthreads();
         // pixels carried over from earlier iterations
         for (int k = tid; k < n_old; k += BT) {
@@ -553,29 +575,27 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             for (int i = tid; i < nt; i += BT) fr[(b0 + i / TW) * kCols + tx0 + i % TW] = tile[i];
         }
 
-        // ---- counters over the band's tile pixels, weighted by visits (order-free)
+        // ---- counters over the band's tile pixels outside the cluster rect,
+        //      weighted by visits (order-free); nothing is observed there, so a
+        //      rendered pixel counts in n_rendered and n_tr.  The pixels inside
+        //      the rect are counted by the error scan below, once per visit.
         const int32_t dq = BT / TW, dr = BT % TW;
         int32_t cy = b0 + tid / TW, cx = tx0 + tid % TW;
         for (int i = tid; i < nt; i += BT, cx += dr, cy += dq) {
             if (cx >= tx1) { cx -= TW; ++cy; }
             const int32_t y = cy, x = cx;
-            const int32_t zr = tile[i];
-            const bool in_rect = x >= rx0 && x < rx1 && y >= ry0 && y < ry1;
-            const int32_t zk = in_rect ? (int32_t)obs[(y - ry0) * rw + (x - rx0)] : 0;
-            if (zr == QPSO_EMPTY && zk == 0) continue;
+            if (tile[i] == QPSO_EMPTY || (x >= rx0 && x < rx1 && y >= ry0 && y < ry1)) continue;
             const uint32_t w = (uint32_t)vis.at(x - minX, y - minY);
-            if (zr != QPSO_EMPTY) {
-                n_rend += w;
-                if (zk == 0) n_tr += w;
-            } else {
-                n_seg += w;
-            }
+            n_rend += w;
+            n_tr += w;
         }
 
         // ---- squared error: the only order-sensitive sum.  Slot t & 2047
         //      accumulates in increasing t (main.cu:693-717); contributions come
         //      only from tile pixels inside the cluster rect, so scan the t whose
         //      row lies in this band and whose pixel lies in that intersection.
+        //      Each t is one visit, so the counters of those pixels are summed
+        //      here too.
         const int32_t qx0 = max(tx0, rx0), qx1 = min(tx1, rx1);
         const int32_t qy0 = max(b0, ry0), qy1 = min(b1, ry1);
         if (qx0 < qx1 && qy0 < qy1) {
@@ -594,9 +614,16 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
                     if (px < qx0 || px >= qx1 || py < qy0 || py >= qy1) continue;
                     const int32_t zr = tile[(py - b0) * TW + px - tx0];
                     const int32_t zk = (int32_t)obs[(py - ry0) * rw + (px - rx0)];
-                    if (zr != QPSO_EMPTY && zk != 0) {
-                        const float e = (float)wsub(zr, zk) * 0.001f;
-                        acc_e[j] = __builtin_fmaf(e, e, acc_e[j]);
+                    if (zr != QPSO_EMPTY) {
+                        ++n_rend;
+                        if (zk == 0) {
+                            ++n_tr;
+                        } else {
+                            const float e = (float)wsub(zr, zk) * 0.001f;
+                            acc_e[j] = __builtin_fmaf(e, e, acc_e[j]);
+                        }
+                    } else if (zk != 0) {
+                        ++n_seg;
                     }
                 }
             }
