@@ -100,7 +100,9 @@ def batched_section(ctx, reps=3):
             "speedup": t_p / t_b, "identical_results": bool(same)}
 
 
-def served_section(ctx, lam=1000.0, seconds=3.0, seed=0):
+def served_section(ctx, lams=(200.0, 1000.0, 4000.0), seconds=1.5, seed=0, slots=4096):
+    """Poisson arrivals of (frame, cluster) requests into the continuous-batching
+    server, one run per arrival rate; latency is submit -> result."""
     meshes = W.config3_meshes(seed)
     mids = [ctx.upload_mesh((m * np.float32(1e-3)).astype(np.float32)) for m, _ in meshes]
     rng = np.random.default_rng(seed)
@@ -111,31 +113,36 @@ def served_section(ctx, lam=1000.0, seconds=3.0, seed=0):
             if scene.clusters[j] is not None:
                 pool.append((mids[mi], scene.cluster(j)))
     params = qpso.make_params(128, 60, w=0.7, c1=1.5, c2=1.5, cam=W.CAM, seed=0)
-    sv = qpso.Server(ctx, 32 << 30)
-    n_req = int(lam * seconds)
-    gaps = rng.exponential(1.0 / lam, n_req)
-    tickets = []
-    t0 = time.perf_counter()
-    t_next = t0
-    for k in range(n_req):
-        t_next += gaps[k]
-        while True:
-            now = time.perf_counter()
-            if now >= t_next:
-                break
-            time.sleep(min(0.0005, t_next - now))
-        mid, cl = pool[k % len(pool)]
-        tickets.append(sv.submit(mid, cl, params))
-    for tk in tickets:
-        sv.wait(tk)
-    st = sv.stats()
-    sv.close()
+    runs = []
+    for lam in lams:
+        sv = qpso.Server(ctx, 32 << 30)
+        sv.set_slots(slots)
+        n_req = int(lam * seconds)
+        gaps = rng.exponential(1.0 / lam, n_req)
+        tickets = []
+        t0 = time.perf_counter()
+        t_next = t0
+        for k in range(n_req):
+            t_next += gaps[k]
+            while True:
+                now = time.perf_counter()
+                if now >= t_next:
+                    break
+                time.sleep(min(0.0005, t_next - now))
+            mid, cl = pool[k % len(pool)]
+            tickets.append(sv.submit(mid, cl, params))
+        for tk in tickets:
+            sv.wait(tk)
+        st = sv.stats()
+        sv.close()
+        runs.append({"lambda_per_s": lam, "requests": n_req, "completed": int(st.completed),
+                     "solves_per_s": st.solves_per_s, "p50_ms": st.p50_ms, "p99_ms": st.p99_ms,
+                     "mean_resident_solves": st.mean_active})
     for m in mids:
         ctx.release_mesh(m)
-    return {"workload": f"Poisson arrivals at {lam:.0f}/s for {seconds:.0f} s, 8 resident meshes "
-                        f"1k..50k faces, 128 particles, 60 iterations, 32 GiB budget",
-            "completed": int(st.completed), "solves_per_s": st.solves_per_s, "p50_ms": st.p50_ms,
-            "p99_ms": st.p99_ms, "mean_resident": st.mean_active}
+    return {"workload": f"Poisson arrivals for {seconds:.1f} s per rate, 8 resident meshes 1k..50k faces, "
+                        f"128 particles, 60 iterations, 32 GiB budget, {slots} particle slots",
+            "runs": runs}
 
 
 def main():
@@ -147,7 +154,7 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=10)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
-    ap.add_argument("--lam", type=float, default=1000.0)
+    ap.add_argument("--lam", type=str, default="200,1000,4000", help="arrival rates of the served section")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -208,7 +215,7 @@ def main():
     if not args.no_batched:
         out["batched"] = batched_section(ctx)
     if not args.no_served:
-        out["served"] = served_section(ctx, lam=args.lam)
+        out["served"] = served_section(ctx, lams=[float(x) for x in args.lam.split(",")])
     ctx.close()
     print(json.dumps(out))
 

@@ -152,9 +152,10 @@ int qpso_solve_run_batch(qpso_ctx* ctx, int n, qpso_solve* const* solves, qpso_r
 int qpso_solve_eval_poses(qpso_solve* s, int n, const float* poses,
                           float* fit, int32_t* aabb, int32_t* frames);
 
-/* Serving: a host thread admits solves as they arrive, under an HBM budget,
- * and steps every admitted solve one iteration per device step (continuous
- * batching).  Inputs are copied at submit. */
+/* Serving: a host thread admits solves as they arrive, under an HBM budget
+ * and a number of particle slots, and steps every admitted solve one
+ * iteration per device step (continuous batching): a request that arrives
+ * while others run joins them at the next step.  Inputs are copied at submit. */
 typedef struct {
     uint64_t submitted, completed, rejected;
     double p50_ms, p99_ms, mean_ms;  /* submit -> result latency */
@@ -164,6 +165,12 @@ typedef struct {
 } qpso_server_stats;
 
 int qpso_server_create(qpso_ctx* ctx, uint64_t hbm_budget_bytes, qpso_server** out);
+/* Slots: the most particles resident at once (default 4096).  A step runs
+ * one iteration of every resident solve, so fewer slots mean shorter steps
+ * and lower latency, more slots fuller launches and higher throughput.  A
+ * solve with more particles than the slots runs alone.  No counterpart in the
+ * reference (it solves one cluster at a time). */
+int qpso_server_set_slots(qpso_server* sv, int64_t max_resident_particles);
 int qpso_server_submit(qpso_server* sv, uint32_t mesh_id, const qpso_cluster* cl,
                        const qpso_params* p, uint64_t* ticket);
 int qpso_server_poll(qpso_server* sv, uint64_t ticket, qpso_result* out);

@@ -696,6 +696,8 @@ struct qpso_server {
     qpso_ctx* ctx = nullptr;
     uint64_t budget = 0;
     uint64_t used = 0;
+    int64_t max_particles = 4096;  // resident particles (slots)
+    int64_t resident = 0;
     hipStream_t st = nullptr;
     std::thread th;
     std::mutex mu;
@@ -714,6 +716,7 @@ struct qpso_server {
     struct Active {
         Req req;
         qpso_solve* s;
+        int iter;  // iterations enqueued so far (every step advances each resident solve by one)
     };
     std::vector<Active> active;
     std::unordered_map<uint64_t, qpso_result> results;
@@ -726,7 +729,14 @@ struct qpso_server {
 
 namespace {
 
+// One device step is one PSO iteration of every resident solve: a fused
+// render+fitness launch over all their particles and one PSO launch.  The
+// host knows each solve's iteration count, so steps are enqueued without a
+// round trip; at most kInFlight steps are queued ahead of the device, so a
+// request that arrives joins within about two steps.  The stream is synchronised
+// only when a solve completes and its result is read.
 void server_loop(qpso_server* sv) {
+    constexpr int kInFlight = 2;
     qpso_ctx* ctx = sv->ctx;
     hipSetDevice(ctx->device);
     QpsoSolveDev* d_descs = nullptr;
@@ -734,20 +744,30 @@ void server_loop(qpso_server* sv) {
     size_t cap_descs = 0, cap_items = 0;
     std::vector<QpsoSolveDev> descs;
     std::vector<QpsoItem> items;
+    hipEvent_t ev[kInFlight];
+    for (auto& e : ev) hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    uint64_t launched = 0;
+    bool changed = true;
     for (;;) {
         std::vector<qpso_server::Req> admit;
         {
             std::unique_lock<std::mutex> lk(sv->mu);
             sv->cv.wait(lk, [&] { return sv->stop || !sv->pending.empty() || !sv->active.empty(); });
             if (sv->stop && sv->active.empty()) break;
-            // admit in arrival order while the budget allows
-            while (!sv->pending.empty() && sv->used + sv->pending.front().bytes <= sv->budget) {
-                sv->used += sv->pending.front().bytes;
+            // admit in arrival order while the budget and the slots allow (a
+            // solve larger than the slots runs when nothing else is resident)
+            while (!sv->pending.empty()) {
+                const auto& r = sv->pending.front();
+                const bool fits = sv->used + r.bytes <= sv->budget &&
+                                  (sv->resident + r.p.n_particles <= sv->max_particles ||
+                                   (sv->resident == 0 && admit.empty()));
+                if (!fits) break;
+                sv->used += r.bytes;
+                sv->resident += r.p.n_particles;
                 admit.push_back(std::move(sv->pending.front()));
                 sv->pending.pop_front();
             }
         }
-        std::vector<qpso_server::Active> fresh;
         for (auto& r : admit) {
             r.cl.depth = r.depth.data();
             qpso_solve* s = nullptr;
@@ -758,49 +778,57 @@ void server_loop(qpso_server* sv) {
                 res.status = rc;
                 sv->results[r.ticket] = res;
                 sv->used -= r.bytes;
+                sv->resident -= r.p.n_particles;
                 ++sv->rejected;
                 sv->done_cv.notify_all();
                 continue;
             }
-            fresh.push_back({std::move(r), s});
+            // InitQPSO on the server stream, ahead of the solve's first step
+            qpso_launch::init(s->d_desc, 1, sv->st);
+            sv->active.push_back({std::move(r), s, 0});
+            changed = true;
         }
-        // new solves: InitQPSO on the server stream, then they join the step
-        for (auto& a : fresh) qpso_launch::init(a.s->d_desc, 1, sv->st);
-        for (auto& a : fresh) sv->active.push_back(std::move(a));
         if (sv->active.empty()) continue;
 
-        // one device step = one PSO iteration of every resident solve
-        descs.clear();
-        items.clear();
-        for (size_t i = 0; i < sv->active.size(); ++i) {
-            descs.push_back(sv->active[i].s->h);
-            for (int q = 0; q < sv->active[i].s->h.N; ++q) items.push_back(QpsoItem{(int32_t)i, q});
+        if (changed) {  // the copies are stream-ordered behind the steps that read the old arrays
+            descs.clear();
+            items.clear();
+            for (size_t i = 0; i < sv->active.size(); ++i) {
+                descs.push_back(sv->active[i].s->h);
+                for (int q = 0; q < sv->active[i].s->h.N; ++q) items.push_back(QpsoItem{(int32_t)i, q});
+            }
+            if (descs.size() > cap_descs || items.size() > cap_items) {
+                hipStreamSynchronize(sv->st);
+                if (descs.size() > cap_descs) {
+                    if (d_descs) hipFree(d_descs);
+                    cap_descs = descs.size() * 2;
+                    hipMalloc(&d_descs, sizeof(QpsoSolveDev) * cap_descs);
+                }
+                if (items.size() > cap_items) {
+                    if (d_items) hipFree(d_items);
+                    cap_items = items.size() * 2;
+                    hipMalloc(&d_items, sizeof(QpsoItem) * cap_items);
+                }
+            }
+            hipMemcpyAsync(d_descs, descs.data(), sizeof(QpsoSolveDev) * descs.size(), hipMemcpyHostToDevice, sv->st);
+            hipMemcpyAsync(d_items, items.data(), sizeof(QpsoItem) * items.size(), hipMemcpyHostToDevice, sv->st);
+            changed = false;
         }
-        if (descs.size() > cap_descs) {
-            if (d_descs) hipFree(d_descs);
-            cap_descs = descs.size() * 2;
-            hipMalloc(&d_descs, sizeof(QpsoSolveDev) * cap_descs);
-        }
-        if (items.size() > cap_items) {
-            if (d_items) hipFree(d_items);
-            cap_items = items.size() * 2;
-            hipMalloc(&d_items, sizeof(QpsoItem) * cap_items);
-        }
-        hipMemcpyAsync(d_descs, descs.data(), sizeof(QpsoSolveDev) * descs.size(), hipMemcpyHostToDevice, sv->st);
-        hipMemcpyAsync(d_items, items.data(), sizeof(QpsoItem) * items.size(), hipMemcpyHostToDevice, sv->st);
+        if (launched >= (uint64_t)kInFlight) hipEventSynchronize(ev[launched % kInFlight]);
         qpso_launch::render_fitness(ctx->rf_cfg, d_descs, d_items, (int)items.size(), sv->st);
         qpso_launch::pso_step(d_descs, (int)descs.size(), sv->st);
-        std::vector<int32_t> iters(sv->active.size());
-        for (size_t i = 0; i < sv->active.size(); ++i)
-            hipMemcpyAsync(&iters[i], sv->active[i].s->h.iter, sizeof(int32_t), hipMemcpyDeviceToHost, sv->st);
-        hipStreamSynchronize(sv->st);
+        hipEventRecord(ev[launched % kInFlight], sv->st);
+        ++launched;
         ++sv->steps;
         sv->active_sum += (double)sv->active.size();
 
+        bool any_done = false;
+        for (auto& a : sv->active) any_done |= ++a.iter >= a.s->h.I;
+        if (!any_done) continue;
+        hipStreamSynchronize(sv->st);
         std::vector<qpso_server::Active> keep;
-        for (size_t i = 0; i < sv->active.size(); ++i) {
-            auto& a = sv->active[i];
-            if (iters[i] < a.s->h.I) { keep.push_back(std::move(a)); continue; }
+        for (auto& a : sv->active) {
+            if (a.iter < a.s->h.I) { keep.push_back(std::move(a)); continue; }
             qpso_result res{};
             const int rc = read_result(a.s, &res, sv->st);
             if (rc && res.status == QPSO_OK) res.status = rc;
@@ -810,12 +838,16 @@ void server_loop(qpso_server* sv) {
             sv->results[a.req.ticket] = res;
             sv->latencies_ms.push_back(std::chrono::duration<double, std::milli>(now - a.req.t_submit).count());
             sv->used -= a.req.bytes;
+            sv->resident -= a.req.p.n_particles;
             ++sv->completed;
             sv->t_last = now;
             sv->done_cv.notify_all();
         }
         sv->active.swap(keep);
+        changed = true;
     }
+    hipStreamSynchronize(sv->st);
+    for (auto& e : ev) hipEventDestroy(e);
     if (d_descs) hipFree(d_descs);
     if (d_items) hipFree(d_items);
 }
@@ -836,6 +868,14 @@ int qpso_server_create(qpso_ctx* ctx, uint64_t hbm_budget_bytes, qpso_server** o
     }
     sv->th = std::thread(server_loop, sv);
     *out = sv;
+    return QPSO_OK;
+}
+
+int qpso_server_set_slots(qpso_server* sv, int64_t max_resident_particles) {
+    if (!sv || max_resident_particles <= 0) return QPSO_E_INVAL;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->max_particles = max_resident_particles;
+    sv->cv.notify_all();
     return QPSO_OK;
 }
 

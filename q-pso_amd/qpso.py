@@ -24,7 +24,7 @@ EXPORTS = [
     "qpso_mesh_upload", "qpso_mesh_release", "qpso_estimate_pose", "qpso_solve_create",
     "qpso_solve_run", "qpso_solve_get_trace", "qpso_solve_destroy", "qpso_solve_run_timed",
     "qpso_solve_run_batch",
-    "qpso_solve_eval_poses", "qpso_server_create", "qpso_server_submit", "qpso_server_poll",
+    "qpso_solve_eval_poses", "qpso_server_create", "qpso_server_set_slots", "qpso_server_submit", "qpso_server_poll",
     "qpso_server_wait", "qpso_server_stats_get", "qpso_server_destroy", "qpso_solve_bytes",
     "qpso_mesh_icosphere", "qpso_mesh_uv_ellipsoid", "qpso_random_quat", "qpso_scene_render",
     "qpso_scene_cluster", "qpso_read_obj", "qpso_read_dpt", "qpso_write_dpt", "qpso_free",
@@ -111,6 +111,7 @@ def lib():
             "qpso_solve_run_batch": (I32, [P, C.c_int, P, P]),
             "qpso_solve_eval_poses": (I32, [P, C.c_int, P, P, P, P]),
             "qpso_server_create": (I32, [P, U64, P]),
+            "qpso_server_set_slots": (I32, [P, C.c_int64]),
             "qpso_server_submit": (I32, [P, U32, P, P, P]),
             "qpso_server_poll": (I32, [P, U64, P]),
             "qpso_server_wait": (I32, [P, U64, P]),
@@ -287,6 +288,9 @@ class Server:
         h = C.c_void_p()
         _check(lib().qpso_server_create(ctx.handle, int(budget_bytes), C.byref(h)), ctx)
         self.handle, self.ctx = h, ctx
+
+    def set_slots(self, max_resident_particles):
+        _check(lib().qpso_server_set_slots(self.handle, int(max_resident_particles)))
 
     def submit(self, mesh_id, cluster, params):
         t = C.c_uint64()

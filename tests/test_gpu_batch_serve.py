@@ -1,0 +1,102 @@
+"""Batched and served forms against the plain form, through the C ABI.
+
+Every solve is independent and deterministic, so a solve run alone, in a
+batch with the other clusters of its frame, or in the server's mix with
+requests that arrive while it runs must give the same result bit for bit:
+gbest pose, fitness and the exact work counters.
+"""
+import time
+
+import numpy as np
+import pytest
+
+import qpso
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = qpso.Context(0)
+    yield c
+    c.close()
+
+
+def same(a, b):
+    return (np.array_equal(np.array(a.pose[:], np.float32).view(np.uint32),
+                           np.array(b.pose[:], np.float32).view(np.uint32))
+            and np.float32(a.fit).view(np.uint32) == np.float32(b.fit).view(np.uint32)
+            and a.raster_px == b.raster_px and a.visit_px == b.visit_px
+            and a.n_degenerate == b.n_degenerate and a.status == b.status)
+
+
+@pytest.fixture(scope="module")
+def frame(ctx):
+    """A frame of uneven solves: 6 objects, 500..20k faces, 64..512 particles, 12 iterations."""
+    scene, plist = W.config2(0, n_objects=6, n_iters=12)
+    meshes = [ctx.upload_mesh(scene.mesh_m(i)) for i in range(len(scene.objects))]
+    jobs = [(meshes[i], scene.cluster(i), p) for i, p in enumerate(plist) if scene.clusters[i] is not None]
+    plain = []
+    for mid, cl, p in jobs:
+        with ctx.solve(mid, cl, p) as s:
+            plain.append(s.run())
+    yield jobs, plain
+    for m in meshes:
+        ctx.release_mesh(m)
+
+
+def test_batch_equals_plain(ctx, frame):
+    jobs, plain = frame
+    solves = [ctx.solve(mid, cl, p) for mid, cl, p in jobs]
+    try:
+        for rep in range(2):  # first call captures the graph, the second replays it
+            res = ctx.run_batch(solves)
+            for k, (a, b) in enumerate(zip(res, plain)):
+                assert same(a, b), f"replay {rep}, solve {k}"
+        # a plain run between replays leaves the batch unaffected
+        assert same(solves[0].run(), plain[0])
+        res = ctx.run_batch(solves)
+        assert all(same(a, b) for a, b in zip(res, plain))
+    finally:
+        for s in solves:
+            s.close()
+
+
+@pytest.mark.parametrize("slots", [256, 100000])
+def test_server_result_same_alone_and_in_mix(ctx, frame, slots):
+    jobs, plain = frame
+    sv = qpso.Server(ctx, 8 << 30)
+    try:
+        sv.set_slots(slots)
+        tickets = []
+        # the first two run alone for a moment, the rest arrive while they run
+        for k in range(2):
+            tickets.append((k, sv.submit(*jobs[k])))
+        time.sleep(0.01)
+        for k in range(2, len(jobs)):
+            tickets.append((k, sv.submit(*jobs[k])))
+            time.sleep(0.002)
+        # the same requests once more, interleaved with the first round's tail
+        for k in range(len(jobs)):
+            tickets.append((k, sv.submit(*jobs[k])))
+        for k, t in tickets:
+            r = sv.wait(t)
+            assert same(r, plain[k]), f"request for solve {k}"
+        st = sv.stats()
+        assert st.completed == len(tickets)
+        if slots > sum(p.n_particles for _, _, p in jobs):
+            assert st.mean_active > 1.0  # requests overlapped on the device
+    finally:
+        sv.close()
+
+
+def test_server_rejects_over_budget(ctx, frame):
+    jobs, _ = frame
+    sv = qpso.Server(ctx, 1 << 10)
+    try:
+        with pytest.raises(qpso.QpsoError) as e:
+            sv.submit(*jobs[0])
+        assert e.value.code == qpso.E_BUDGET
+    finally:
+        sv.close()
