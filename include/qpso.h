@@ -144,6 +144,26 @@ int qpso_solve_run_timed(qpso_solve* s, qpso_result* out, qpso_timing* tm);
  * to running each solve alone. */
 int qpso_solve_run_batch(qpso_ctx* ctx, int n, qpso_solve* const* solves, qpso_result* out);
 
+/* One measured candidate of qpso_autotune. */
+typedef struct {
+    int32_t config;      /* render+fitness configuration index */
+    int32_t threads;     /* workgroup size (one workgroup per particle) */
+    int32_t tile_words;  /* 32-bit LDS words per workgroup: vertex cache, face queues, depth tile */
+    int32_t big_face;    /* faces over this many clipped pixels are drawn by a whole wave */
+    double ms;           /* InitQPSO + `iters` iterations of the listed solves */
+} qpso_tune_entry;
+
+/* Autotuner: times InitQPSO + `iters` iterations of the listed solves, run
+ * together as in a batch, under every workgroup/tile configuration and
+ * big-face threshold, and stores the fastest in each solve for its later
+ * plain, timed, stage and batched runs.  Results do not depend on the
+ * choice.  `table` (nullable when cap is 0) receives up to `cap` candidates,
+ * `*n_out` (nullable) their number.  The swarm state is left undefined; every
+ * run starts with InitQPSO.  No counterpart in the reference, whose launch
+ * shapes are compile-time constants (main.cu:1607, main.h:84-99). */
+int qpso_autotune(qpso_ctx* ctx, int n, qpso_solve* const* solves, int iters,
+                  qpso_tune_entry* table, int cap, int* n_out);
+
 /* Stage entry point (RenderFixedPointKernelAABB + ComputeAABBFinalReduction +
  * ComputeFitnessKernel, main.cu:1633-1681) on caller-given poses [7][n]
  * (n <= N), starting from empty depth buffers.  Outputs (nullable):

@@ -182,8 +182,10 @@ struct qpso_ctx {
         QpsoSolveDev* d_descs = nullptr;
         QpsoItem* d_items = nullptr;
         int n_items = 0;
+        int rf_cfg = 0;
         hipGraphExec_t exec = nullptr;
     } plan;
+    int big_face = 96;  // default big-face threshold of new solves
 };
 
 struct qpso_solve {
@@ -196,6 +198,9 @@ struct qpso_solve {
     qpso_params params{};
     int32_t rect[4]{};
     uint32_t mesh_id = 0;
+    int rf_cfg = 0;                  // render+fitness configuration (qpso_autotune)
+    hipGraphExec_t exec = nullptr;   // captured plain run: init + I x (render+fitness, PSO)
+    std::vector<hipEvent_t> timed_ev;     // qpso_solve_run_timed's events, kept with the solve
 };
 
 namespace {
@@ -273,10 +278,36 @@ int launch_solve(qpso_solve* s, hipStream_t st) {
     HIP_TRY(ctx, qpso_launch::init(s->d_desc, 1, st));
     const QpsoItem* items = reinterpret_cast<const QpsoItem*>(s->arena + s->L.items);
     for (int it = 0; it < s->h.I; ++it) {
-        HIP_TRY(ctx, qpso_launch::render_fitness(ctx->rf_cfg, s->d_desc, items, s->h.N, st));
+        HIP_TRY(ctx, qpso_launch::render_fitness(s->rf_cfg, s->d_desc, items, s->h.N, st));
         HIP_TRY(ctx, qpso_launch::pso_step(s->d_desc, 1, st));
     }
     return QPSO_OK;
+}
+
+// The plain run as one graph launch (captured at the first run, and again
+// after the configuration changes): 2I + 1 kernels without host launch cost.
+int run_solve_graph(qpso_solve* s, hipStream_t st) {
+    qpso_ctx* ctx = s->ctx;
+    if (!s->exec) {
+        hipGraph_t graph = nullptr;
+        HIP_TRY(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int rc = launch_solve(s, st);
+        const hipError_t e = hipStreamEndCapture(st, &graph);
+        if (rc || e != hipSuccess) {
+            if (graph) hipGraphDestroy(graph);
+            return rc ? rc : set_err(ctx, QPSO_E_HIP, "solve graph capture failed");
+        }
+        const hipError_t e2 = hipGraphInstantiate(&s->exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (e2 != hipSuccess) { s->exec = nullptr; return set_err(ctx, QPSO_E_HIP, "solve graph instantiate failed"); }
+    }
+    HIP_TRY(ctx, hipGraphLaunch(s->exec, st));
+    return QPSO_OK;
+}
+
+void drop_solve_graph(qpso_solve* s) {
+    if (s->exec) hipGraphExecDestroy(s->exec);
+    s->exec = nullptr;
 }
 
 }  // namespace
@@ -451,6 +482,8 @@ int qpso_solve_create(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl, c
     h.stale_cnt[0] = reinterpret_cast<int32_t*>(A + s->L.cnt0);
     h.stale_cnt[1] = reinterpret_cast<int32_t*>(A + s->L.cnt1);
     h.stale_cap = kStaleCap;
+    h.big_face = ctx->big_face;
+    s->rf_cfg = ctx->rf_cfg;
     h.flags = reinterpret_cast<int32_t*>(A + s->L.flags);
     h.counters = reinterpret_cast<unsigned long long*>(A + s->L.counters);
     if (s->trace) {
@@ -487,6 +520,8 @@ int qpso_solve_destroy(qpso_solve* s) {
     hipStreamSynchronize(ctx->stream);
     auto& P = ctx->plan;
     if (std::find(P.solves.begin(), P.solves.end(), s) != P.solves.end()) invalidate_plan(ctx);
+    drop_solve_graph(s);
+    for (auto& e : s->timed_ev) hipEventDestroy(e);
     hipFree(s->arena);
     delete s;
     return QPSO_OK;
@@ -497,7 +532,7 @@ int qpso_solve_run(qpso_solve* s, qpso_result* out) {
     qpso_ctx* ctx = s->ctx;
     std::lock_guard<std::mutex> g(ctx->mu);
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    if (int r = launch_solve(s, ctx->stream)) return r;
+    if (int r = run_solve_graph(s, ctx->stream)) return r;
     return read_result(s, out, ctx->stream);
 }
 
@@ -508,21 +543,29 @@ int qpso_solve_run_timed(qpso_solve* s, qpso_result* out, qpso_timing* tm) {
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     const int I = s->h.I;
-    std::vector<hipEvent_t> ev(2 + 2 * (size_t)I);
-    for (auto& e : ev) HIP_TRY(ctx, hipEventCreate(&e));
-    auto destroy = [&]() { for (auto& e : ev) hipEventDestroy(e); };
-    const QpsoItem* items = reinterpret_cast<const QpsoItem*>(s->arena + s->L.items);
-    hipError_t e = hipEventRecord(ev[0], st);
-    if (e == hipSuccess) e = qpso_launch::init(s->d_desc, 1, st);
-    if (e == hipSuccess) e = hipEventRecord(ev[1], st);
-    for (int it = 0; it < I && e == hipSuccess; ++it) {
-        e = qpso_launch::render_fitness(ctx->rf_cfg, s->d_desc, items, s->h.N, st);
-        if (e == hipSuccess) e = hipEventRecord(ev[2 + 2 * it], st);
-        if (e == hipSuccess) e = qpso_launch::pso_step(s->d_desc, 1, st);
-        if (e == hipSuccess) e = hipEventRecord(ev[3 + 2 * it], st);
+    // an event between every two kernels, launched on the stream (event
+    // records captured into a graph do not time on this runtime)
+    if (s->timed_ev.size() != 2 + 2 * (size_t)I) {
+        s->timed_ev.resize(2 + 2 * (size_t)I);
+        for (auto& e : s->timed_ev) HIP_TRY(ctx, hipEventCreate(&e));
     }
+    const std::vector<hipEvent_t>& ev = s->timed_ev;
+    const QpsoItem* items = reinterpret_cast<const QpsoItem*>(s->arena + s->L.items);
+    auto enqueue = [&]() {
+        hipError_t e = hipEventRecord(ev[0], st);
+        if (e == hipSuccess) e = qpso_launch::init(s->d_desc, 1, st);
+        if (e == hipSuccess) e = hipEventRecord(ev[1], st);
+        for (int it = 0; it < I && e == hipSuccess; ++it) {
+            e = qpso_launch::render_fitness(s->rf_cfg, s->d_desc, items, s->h.N, st);
+            if (e == hipSuccess) e = hipEventRecord(ev[2 + 2 * it], st);
+            if (e == hipSuccess) e = qpso_launch::pso_step(s->d_desc, 1, st);
+            if (e == hipSuccess) e = hipEventRecord(ev[3 + 2 * it], st);
+        }
+        return e;
+    };
+    hipError_t e = enqueue();
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) { destroy(); return set_err(ctx, QPSO_E_HIP, std::string("timed run: ") + hipGetErrorString(e)); }
+    if (e != hipSuccess) return set_err(ctx, QPSO_E_HIP, std::string("timed run: ") + hipGetErrorString(e));
     float ms = 0;
     std::memset(tm, 0, sizeof(*tm));
     hipEventElapsedTime(&ms, ev[0], ev[1]);
@@ -536,7 +579,6 @@ int qpso_solve_run_timed(qpso_solve* s, qpso_result* out, qpso_timing* tm) {
     hipEventElapsedTime(&ms, ev[0], ev[1 + 2 * I]);
     tm->ms_solve = ms;
     tm->launches = I;
-    destroy();
     return read_result(s, out, st);
 }
 
@@ -624,7 +666,7 @@ int qpso_solve_eval_poses(qpso_solve* s, int n, const float* poses, float* fit, 
     tmp.dbg_aabb = d_aabb;
     if (hipMalloc(&d_tmp, sizeof(QpsoSolveDev)) != hipSuccess) { cleanup(); return QPSO_E_NOMEM; }
     hipMemcpyAsync(d_tmp, &tmp, sizeof(tmp), hipMemcpyHostToDevice, st);
-    hipError_t e = qpso_launch::render_fitness(ctx->rf_cfg, d_tmp,
+    hipError_t e = qpso_launch::render_fitness(s->rf_cfg, d_tmp,
                                                reinterpret_cast<const QpsoItem*>(A + s->L.items), n, st);
     if (e == hipSuccess && fit) e = hipMemcpyAsync(fit, A + s->L.fit, sizeof(float) * n, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess && aabb) e = hipMemcpyAsync(aabb, d_aabb, sizeof(int32_t) * 4 * n, hipMemcpyDeviceToHost, st);
@@ -633,6 +675,94 @@ int qpso_solve_eval_poses(qpso_solve* s, int n, const float* poses, float* fit, 
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     cleanup();
     if (e != hipSuccess) return set_err(ctx, QPSO_E_HIP, std::string("eval_poses: ") + hipGetErrorString(e));
+    return QPSO_OK;
+}
+
+// ---- autotuner --------------------------------------------------------------
+int qpso_autotune(qpso_ctx* ctx, int n, qpso_solve* const* solves, int iters, qpso_tune_entry* table, int cap,
+                  int* n_out) {
+    if (!ctx || n <= 0 || !solves || iters <= 0 || (cap > 0 && !table)) return QPSO_E_INVAL;
+    for (int i = 0; i < n; ++i)
+        if (!solves[i] || solves[i]->ctx != ctx) return QPSO_E_INVAL;
+    static const int kBigFaces[] = {48, 96, 160};
+    const int n_cfg = qpso_launch::rf_num_configs();
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    // the solves run together as in a batch: descriptors with the candidate
+    // threshold, particles of the heaviest solves first
+    std::vector<QpsoSolveDev> descs(n);
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) { descs[i] = solves[i]->h; order[i] = i; }
+    auto cost = [&](int i) { return (double)descs[i].F_eff * descs[i].N; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+    std::vector<QpsoItem> items;
+    for (int i : order)
+        for (int q = 0; q < descs[i].N; ++q) items.push_back(QpsoItem{i, q});
+    QpsoSolveDev* d_descs = nullptr;
+    QpsoItem* d_items = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+        hipStreamSynchronize(st);
+        if (d_descs) hipFree(d_descs);
+        if (d_items) hipFree(d_items);
+        if (e0) hipEventDestroy(e0);
+        if (e1) hipEventDestroy(e1);
+    };
+    if (hipMalloc(&d_descs, sizeof(QpsoSolveDev) * n) != hipSuccess ||
+        hipMalloc(&d_items, sizeof(QpsoItem) * items.size()) != hipSuccess) {
+        cleanup();
+        return set_err(ctx, QPSO_E_NOMEM, "autotune scratch allocation failed");
+    }
+    hipError_t e = hipMemcpy(d_items, items.data(), sizeof(QpsoItem) * items.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    auto run = [&](int cfg) {
+        hipError_t r = qpso_launch::init(d_descs, n, st);
+        for (int it = 0; it < iters && r == hipSuccess; ++it) {
+            r = qpso_launch::render_fitness(cfg, d_descs, d_items, (int)items.size(), st);
+            if (r == hipSuccess) r = qpso_launch::pso_step(d_descs, n, st);
+        }
+        return r;
+    };
+    int best_cfg = 0, best_big = kBigFaces[0], k = 0;
+    float best_ms = 0.0f;
+    for (int b = 0; b < 3 && e == hipSuccess; ++b) {
+        for (auto& d : descs) d.big_face = kBigFaces[b];
+        e = hipMemcpy(d_descs, descs.data(), sizeof(QpsoSolveDev) * n, hipMemcpyHostToDevice);
+        if (b == 0 && e == hipSuccess) e = run(0);  // warm-up
+        for (int c = 0; c < n_cfg && e == hipSuccess; ++c) {
+            float ms = 0.0f;
+            for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {  // the faster of two runs
+                e = hipEventRecord(e0, st);
+                if (e == hipSuccess) e = run(c);
+                if (e == hipSuccess) e = hipEventRecord(e1, st);
+                if (e == hipSuccess) e = hipEventSynchronize(e1);
+                float t = 0.0f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+                ms = rep == 0 ? t : std::min(ms, t);
+            }
+            if (e != hipSuccess) break;
+            if (k < cap) {
+                table[k] = qpso_tune_entry{c, qpso_launch::rf_config_threads(c), qpso_launch::rf_config_tile(c),
+                                           kBigFaces[b], (double)ms};
+            }
+            if (k == 0 || ms < best_ms) { best_ms = ms; best_cfg = c; best_big = kBigFaces[b]; }
+            ++k;
+        }
+    }
+    cleanup();
+    if (e != hipSuccess) return set_err(ctx, QPSO_E_HIP, std::string("autotune: ") + hipGetErrorString(e));
+    if (n_out) *n_out = k;
+    // apply the choice; cached graphs hold the old configuration
+    invalidate_plan(ctx);
+    for (int i = 0; i < n; ++i) {
+        qpso_solve* s = solves[i];
+        s->rf_cfg = best_cfg;
+        s->h.big_face = best_big;
+        drop_solve_graph(s);
+        HIP_TRY(ctx, hipMemcpy(s->d_desc, &s->h, sizeof(QpsoSolveDev), hipMemcpyHostToDevice));
+    }
     return QPSO_OK;
 }
 
@@ -645,9 +775,13 @@ int qpso_solve_run_batch(qpso_ctx* ctx, int n, qpso_solve* const* solves, qpso_r
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     auto& P = ctx->plan;
-    const bool same = P.exec && (int)P.solves.size() == n && std::equal(P.solves.begin(), P.solves.end(), solves);
+    // the batch runs with its first solve's configuration
+    const int cfg = solves[0]->rf_cfg;
+    const bool same = P.exec && P.rf_cfg == cfg && (int)P.solves.size() == n &&
+                      std::equal(P.solves.begin(), P.solves.end(), solves);
     if (!same) {
         invalidate_plan(ctx);
+        P.rf_cfg = cfg;
         P.solves.assign(solves, solves + n);
         std::vector<QpsoSolveDev> descs(n);
         std::vector<int> order(n);
@@ -667,7 +801,7 @@ int qpso_solve_run_batch(qpso_ctx* ctx, int n, qpso_solve* const* solves, qpso_r
         HIP_TRY(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         hipError_t e = qpso_launch::init(P.d_descs, n, st);
         for (int it = 0; it < P.max_iters && e == hipSuccess; ++it) {
-            e = qpso_launch::render_fitness(ctx->rf_cfg, P.d_descs, P.d_items, P.n_items, st);
+            e = qpso_launch::render_fitness(cfg, P.d_descs, P.d_items, P.n_items, st);
             if (e == hipSuccess) e = qpso_launch::pso_step(P.d_descs, n, st);
         }
         hipError_t e2 = hipStreamEndCapture(st, &graph);

@@ -27,12 +27,6 @@ namespace {
 
 constexpr int kRows = 480;
 constexpr int kCols = 640;
-// A face whose clipped rectangle has more pixels than this is rasterized by a
-// whole wave, so one large face does not hold its wave's other 63 lanes idle.
-#ifndef QPSO_BIG_FACE
-#define QPSO_BIG_FACE 96
-#endif
-constexpr int kBigFace = QPSO_BIG_FACE;
 
 // Global-memory pointers: loads through them cannot alias LDS, so the compiler
 // may move them across LDS stores and atomics (generic pointers could).
@@ -177,6 +171,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     const int F = S.F, Fe = S.F_eff;
     const QG QpsoFaceRef* __restrict__ faces = (const QG QpsoFaceRef*)S.faces;
     const int n_faces = S.n_faces;
+    const int32_t big_face = S.big_face;
 
     if (tid == 0) {
         s_box[0] = QPSO_AABB_MIN_SENT; s_box[1] = QPSO_AABB_MAX_SENT;
@@ -197,8 +192,9 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     int32_t* __restrict__ vY = lds + Ve;
     float* __restrict__ vZ = reinterpret_cast<float*>(lds + 2 * Ve);
     constexpr int kQCap = CAP / 32;   // small faces needing the per-pixel depth
-    // faces of more than kBigFace pixels: clipped rectangle (x0 | x1 << 16,
-    // y0 | y1 << 16), constant depth, face entry << 1 | flat
+    // faces of more than S.big_face pixels, rasterized by a whole wave so one
+    // large face does not hold its wave's other 63 lanes idle: clipped
+    // rectangle (x0 | x1 << 16, y0 | y1 << 16), constant depth, face << 1 | flat
     constexpr int kBQCap = CAP / 128;
     int32_t* __restrict__ queue = cached ? lds + 3 * Ve : lds;
     int32_t* __restrict__ bq_x = queue + kQCap;
@@ -479,7 +475,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         //      columns).  Faces whose written depth is provably one integer
         //      over the rectangle (flat_depth) store that constant; the others
         //      are queued and rasterized with the per-pixel depth after the
-        //      pass; faces of more than kBigFace pixels go to the wave queue.
+        //      pass; faces of more than big_face pixels go to the wave queue.
         auto draw = [&](int32_t i) {
             Face fc;
             Setup su;
@@ -491,7 +487,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             px_r += (unsigned long long)n;
             int32_t kd = 0, qb;
             const bool flat = flat_depth(fc, su, x0, x1, y0, y1, kd);
-            if (n > kBigFace && (qb = atomicAdd(&s_nqb, 1)) < kBQCap) {
+            if (n > big_face && (qb = atomicAdd(&s_nqb, 1)) < kBQCap) {
                 bq_x[qb] = x0 | (x1 << 16);
                 bq_y[qb] = y0 | (y1 << 16);
                 bq_k[qb] = kd;

@@ -23,7 +23,7 @@ EXPORTS = [
     "qpso_params_default", "qpso_create", "qpso_destroy", "qpso_last_error", "qpso_status_str",
     "qpso_mesh_upload", "qpso_mesh_release", "qpso_estimate_pose", "qpso_solve_create",
     "qpso_solve_run", "qpso_solve_get_trace", "qpso_solve_destroy", "qpso_solve_run_timed",
-    "qpso_solve_run_batch",
+    "qpso_solve_run_batch", "qpso_autotune",
     "qpso_solve_eval_poses", "qpso_server_create", "qpso_server_set_slots", "qpso_server_submit", "qpso_server_poll",
     "qpso_server_wait", "qpso_server_stats_get", "qpso_server_destroy", "qpso_solve_bytes",
     "qpso_mesh_icosphere", "qpso_mesh_uv_ellipsoid", "qpso_random_quat", "qpso_scene_render",
@@ -57,6 +57,11 @@ class Result(C.Structure):
 class Trace(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in
                 ("fit_trace", "gbest_trace", "argmin_trace", "pos", "vel", "pbest", "pbest_fit", "frames")]
+
+
+class TuneEntry(C.Structure):
+    _fields_ = [("config", C.c_int32), ("threads", C.c_int32), ("tile_words", C.c_int32),
+                ("big_face", C.c_int32), ("ms", C.c_double)]
 
 
 class Timing(C.Structure):
@@ -110,6 +115,7 @@ def lib():
             "qpso_solve_run_timed": (I32, [P, P, P]),
             "qpso_solve_run_batch": (I32, [P, C.c_int, P, P]),
             "qpso_solve_eval_poses": (I32, [P, C.c_int, P, P, P, P]),
+            "qpso_autotune": (I32, [P, C.c_int, P, C.c_int, P, C.c_int, P]),
             "qpso_server_create": (I32, [P, U64, P]),
             "qpso_server_set_slots": (I32, [P, C.c_int64]),
             "qpso_server_submit": (I32, [P, U32, P, P, P]),
@@ -224,6 +230,17 @@ class Context:
         res = (Result * len(solves))()
         _check(lib().qpso_solve_run_batch(self.handle, len(solves), arr, res), self)
         return list(res)
+
+    def autotune(self, solves, iters=3):
+        """Time every configuration on these solves (run together) and keep
+        the fastest in each; returns the measured table, fastest first."""
+        arr = (C.c_void_p * len(solves))(*[s.handle for s in solves])
+        table = (TuneEntry * 64)()
+        n = C.c_int()
+        _check(lib().qpso_autotune(self.handle, len(solves), arr, int(iters), table, 64, C.byref(n)), self)
+        rows = [dict(config=t.config, threads=t.threads, tile_words=t.tile_words, big_face=t.big_face, ms=t.ms)
+                for t in table[:min(n.value, 64)]]
+        return sorted(rows, key=lambda r: r["ms"])
 
 
 class Solve:

@@ -100,3 +100,19 @@ def test_server_rejects_over_budget(ctx, frame):
         assert e.value.code == qpso.E_BUDGET
     finally:
         sv.close()
+
+
+def test_autotune_keeps_results(ctx, frame):
+    jobs, plain = frame
+    solves = [ctx.solve(mid, cl, p) for mid, cl, p in jobs]
+    try:
+        table = ctx.autotune(solves, iters=2)
+        assert len(table) == 15 and table[0]["ms"] <= table[-1]["ms"]
+        assert {r["big_face"] for r in table} == {48, 96, 160}
+        res = ctx.run_batch(solves)
+        assert all(same(a, b) for a, b in zip(res, plain))
+        for s, b in zip(solves, plain):
+            assert same(s.run(), b)
+    finally:
+        for s in solves:
+            s.close()
