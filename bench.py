@@ -42,7 +42,7 @@ def pct(v, q):
 
 def traffic_from_profiles():
     """HBM bytes per render+fitness launch measured by rocprofv3 --pmc (committed summary)."""
-    path = os.path.join(REPO, "profiles", "render_fitness_single_solve_10k_faces_pmc.json")
+    path = os.path.join(REPO, "profiles", "k_render_fitness_10k_faces_256_particles_pmc.json")
     if not os.path.exists(path):
         return None
     try:
