@@ -319,7 +319,19 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     ty0 = max(ty0, 0); ty1 = min(ty1, kRows);
     const bool has_tile = tx0 < tx1 && ty0 < ty1;
     const int32_t TW = has_tile ? tx1 - tx0 : 1;
-    const int32_t BH = tcap / TW;  // >= 1: TW <= 640 and tcap >= 7 CAP / 16 - 2 BT - 3 >= 3909
+    // Depth region D: the tile pixels whose depth value the fitness reads, the
+    // tile inside the cluster rect.  Elsewhere only "rendered" matters (a bit
+    // per pixel, the mask), except in the column x = minX and the row y = maxY,
+    // where the scan leaves pixels unvisited and their depth carries to the
+    // next iteration: those two lines keep exact depth in their own arrays.
+    // A stage call that dumps the rendered frames keeps the depth of the whole
+    // tile (D = tile, no mask).
+    const bool full = S.dbg_frames != nullptr;
+    const int32_t dx0 = full ? tx0 : max(tx0, rx0), dx1 = full ? tx1 : min(tx1, rx1);
+    const int32_t dy0 = full ? ty0 : max(ty0, ry0), dy1 = full ? ty1 : min(ty1, ry1);
+    const int32_t DW = dx0 < dx1 && dy0 < dy1 ? dx1 - dx0 : 0;
+    const int32_t MW = full ? 0 : (TW + 31) >> 5;  // mask words per tile row
+    const bool col_line = minX >= tx0 && minX < tx1;
 
     float acc_e[SPT];
 #pragma unroll
@@ -346,8 +358,8 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     // per-pixel depth over [x0, x1) x [y0, y1): edge values at (x0, y0), per
     // pixel CX -= FDY, from a row's last pixel to the next row's first
     // CX += FDX + (w - 1) FDY
-    auto raster_exact = [&](const Face& fc, const Setup& su, int32_t x0, int32_t x1, int32_t y0, int32_t y1,
-                            int32_t b0) {
+    int32_t db0 = 0;  // first D row of the current band
+    auto raster_exact = [&](const Face& fc, const Setup& su, int32_t x0, int32_t x1, int32_t y0, int32_t y1) {
         if (x0 >= x1 || y0 >= y1) return;
         const int32_t w = x1 - x0, n = w * (y1 - y0);
         int32_t CX2 = wsub(wadd(su.CY2, wmul(y0 - fc.miny, su.FDX23)), wmul(x0 - fc.minx, su.FDY23));
@@ -355,8 +367,8 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         const int32_t step2 = wsub(0, su.FDY23), step3 = wsub(0, su.FDY31);
         const int32_t wrap2 = wadd(su.FDX23, wmul(w - 1, su.FDY23));
         const int32_t wrap3 = wadd(su.FDX31, wmul(w - 1, su.FDY31));
-        const int32_t awrap = TW - w + 1;
-        int32_t* addr = tile + (y0 - b0) * TW + (x0 - tx0);
+        const int32_t awrap = DW - w + 1;
+        int32_t* addr = tile + (y0 - db0) * DW + (x0 - dx0);
         int32_t c = w;
         for (int32_t m = 0; m < n; ++m) {
             atomicMin(addr, pixel_depth(CX2, CX3, su));
@@ -373,7 +385,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     // at most.  floor(a / w) for a <= 64, w <= 640 is trunc((a + 0.5) * rcp(w)):
     // (a + 0.5) / w lies at least 1 / 1280 from an integer, far beyond the
     // error of the approximate reciprocal.
-    auto raster_wave = [&](int32_t q, int32_t b0) {
+    auto raster_wave = [&](int32_t q) {
         const int32_t x0 = bq_x[q] & 0xffff, x1 = bq_x[q] >> 16;
         const int32_t y0 = bq_y[q] & 0xffff, y1 = bq_y[q] >> 16;
         const int32_t kd = bq_k[q], fi = bq_i[q];
@@ -382,8 +394,8 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         const float rw = __builtin_amdgcn_rcpf((float)w);
         const int32_t sy = (int32_t)(64.5f * rw), sx = 64 - sy * w;
         int32_t dy = (int32_t)(((float)lane + 0.5f) * rw), dx = lane - dy * w;
-        const int32_t astep = sy * TW + sx, afix = TW - w;
-        int32_t off = (y0 + dy - b0) * TW + (x0 + dx - tx0);
+        const int32_t astep = sy * DW + sx, afix = DW - w;
+        int32_t off = (y0 + dy - db0) * DW + (x0 + dx - dx0);
         if (fi & 1) {  // flat: one depth over the rectangle
             for (int32_t m = lane; m < n; m += 64) {
                 atomicMin(tile + off, kd);
@@ -445,37 +457,87 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         }
     }
 
+    // Band rows [b0, b1): as many tile rows as fit, a row costing its mask
+    // words and its column-line word, plus DW for the rows of D; the row line
+    // (TW words) and four words of alignment come once per band.
+    auto band_end = [&](int32_t b0) {
+        const int32_t cr = MW + 1;
+        int32_t budget = tcap - TW - 8, y = b0;
+        auto take = [&](int32_t y_end, int32_t cost) {
+            if (y < y_end) {
+                const int32_t n = min(y_end - y, budget / cost);
+                y += n;
+                budget -= n * cost;
+            }
+            return y >= y_end;
+        };
+        if (take(min(dy0, ty1), cr) && take(DW > 0 ? dy1 : y, cr + DW)) take(ty1, cr);
+        return max(y, b0 + 1);  // one D row fits: TW <= 640 and tcap >= 3909
+    };
+
+    uint32_t* mask = nullptr;    // [band rows][MW]
+    int32_t* col_d = nullptr;    // [band rows]: depth of (minX, y)
+    int32_t* row_d = nullptr;    // [TW]: depth of (x, maxY)
     uint64_t fmask = ~0ull;  // bit k: face entry tid + k*BT still reaches later bands
     const bool use_mask = n_faces <= 64 * BT;
-    for (int32_t b0 = ty0; has_tile && b0 < ty1; b0 += BH) {
-        const int32_t b1 = min(b0 + BH, ty1);  // band rows [b0, b1)
-        const int32_t nt = (b1 - b0) * TW;
+    for (int32_t b0 = ty0, b1; has_tile && b0 < ty1; b0 = b1) {
+        b1 = band_end(b0);  // band rows [b0, b1)
+        const int32_t nb = b1 - b0;
+        db0 = max(b0, dy0);
+        const int32_t db1 = min(b1, dy1);
+        const int32_t nd = DW > 0 && db0 < db1 ? (db1 - db0) * DW : 0;  // depth words
+        mask = reinterpret_cast<uint32_t*>(tile + ((nd + 3) & ~3));
+        col_d = reinterpret_cast<int32_t*>(mask) + nb * MW;
+        row_d = col_d + nb;
+        const bool row_line = maxY >= b0 && maxY < b1;
         {
             typedef int32_t v4i __attribute__((ext_vector_type(4)));
             v4i* t4 = reinterpret_cast<v4i*>(tile);
             const v4i e4 = {QPSO_EMPTY, QPSO_EMPTY, QPSO_EMPTY, QPSO_EMPTY};
-            for (int i = tid; i < nt / 4; i += BT) t4[i] = e4;
-            if (tid < (nt & 3)) tile[(nt & ~3) + tid] = QPSO_EMPTY;
+            for (int i = tid; i < nd / 4; i += BT) t4[i] = e4;
+            if (tid < (nd & 3)) tile[(nd & ~3) + tid] = QPSO_EMPTY;
+            for (int i = tid; i < nb * MW; i += BT) mask[i] = 0u;
+            for (int i = tid; i < nb + TW; i += BT) col_d[i] = QPSO_EMPTY;
         }
         __syncthreads();
-        // pixels carried over from earlier iterations
+        // pixels carried over from earlier iterations (their values are below 60000)
         for (int k = tid; k < n_old; k += BT) {
             const QpsoStale e = load_stale(old, k);
             const int32_t ex = e.pos & 0xffff, ey = e.pos >> 16;
-            if (ey >= b0 && ey < b1 && ex >= tx0 && ex < tx1) tile[(ey - b0) * TW + ex - tx0] = e.val;
+            if (ey >= b0 && ey < b1 && ex >= tx0 && ex < tx1) {
+                if (ex >= dx0 && ex < dx1 && ey >= db0 && ey < db1) tile[(ey - db0) * DW + ex - dx0] = e.val;
+                if (MW) atomicOr(&mask[(ey - b0) * MW + ((ex - tx0) >> 5)], 1u << ((ex - tx0) & 31));
+                if (ex == minX) col_d[ey - b0] = e.val;
+                if (ey == maxY) row_d[ex - tx0] = e.val;
+            }
         }
         __syncthreads();
 
-        // ---- raster: bbox fill with ds_min_i32 (main.h:1355-1488) in two steps
-        //      per wave.  Cull: the wave scans 64 face entries at a time and
-        //      appends those that are front-facing and reach the band and the
-        //      tile to its ring in LDS.  Draw: whenever 64 are waiting, one per
-        //      lane, each lane walks its face's clipped bbox as one flat pixel
-        //      loop (a nested row/column loop would diverge as max rows x max
-        //      columns).  Faces whose written depth is provably one integer
-        //      over the rectangle (flat_depth) store that constant; the others
-        //      are queued and rasterized with the per-pixel depth after the
-        //      pass; faces of more than big_face pixels go to the wave queue.
+        // bits of the mask row y over columns [x0, x1)
+        auto mark_rows = [&](int32_t x0, int32_t x1, int32_t y0, int32_t y1) {
+            const int32_t w0 = (x0 - tx0) >> 5, w1 = (x1 - 1 - tx0) >> 5;
+            const uint32_t m0 = ~0u << ((x0 - tx0) & 31), m1 = ~0u >> (31 - ((x1 - 1 - tx0) & 31));
+            for (int32_t y = y0; y < y1; ++y) {
+                uint32_t* row = mask + (y - b0) * MW;
+                for (int32_t w = w0; w <= w1; ++w) {
+                    const uint32_t m = (w == w0 ? m0 : ~0u) & (w == w1 ? m1 : ~0u);
+                    atomicOr(&row[w], m);
+                }
+            }
+        };
+
+        // ---- raster: bbox fill (main.h:1355-1488) in two steps per wave.
+        //      Cull: the wave scans 64 face entries at a time and appends those
+        //      that are front-facing and reach the band and the tile to its
+        //      ring in LDS.  Draw: whenever 64 are waiting, one per lane, the
+        //      lane writes its face's clipped bbox C: the part in D with
+        //      ds_min_i32, as one flat pixel loop (a nested row/column loop
+        //      would diverge as max rows x max columns); the rest of C into the
+        //      mask where its depth is below 60000; and the column and row
+        //      lines.  Faces whose depth over the D part is provably one
+        //      integer (flat_depth) store that constant; the others are queued
+        //      and rasterized with the per-pixel depth after the pass; D parts
+        //      of more than big_face pixels go to the wave queue.
         auto draw = [&](int32_t i) {
             Face fc;
             Setup su;
@@ -483,18 +545,49 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             face_setup(fc, su);
             const int32_t x0 = max(fc.minx, tx0), x1 = min(fc.maxx, tx1);
             const int32_t y0 = max(fc.miny, b0), y1 = min(fc.maxy, b1);
-            const int32_t w = x1 - x0, n = w * (y1 - y0);
-            px_r += (unsigned long long)n;
-            int32_t kd = 0, qb;
-            const bool flat = flat_depth(fc, su, x0, x1, y0, y1, kd);
+            px_r += (unsigned long long)((x1 - x0) * (y1 - y0));
+            // part in D
+            const int32_t px0 = max(x0, dx0), px1 = min(x1, dx1), py0 = max(y0, db0), py1 = min(y1, db1);
+            const bool in_d = px0 < px1 && py0 < py1;
+            const bool whole = in_d && px0 == x0 && px1 == x1 && py0 == y0 && py1 == y1;
+            int32_t kd = 0, kmin = 0, kmax = 0;
+            const bool flat = in_d && flat_depth(fc, su, px0, px1, py0, py1, kd);
+            // bounds over all of C, for the mask and the lines
+            bool bounded;
+            if (whole) {
+                bounded = flat;
+                kmin = kmax = kd;
+            } else {
+                bounded = depth_bounds(fc, su, x0, x1, y0, y1, kmin, kmax);
+            }
+            if (MW && !whole) {
+                if (bounded && kmax < QPSO_EMPTY) {
+                    mark_rows(x0, x1, y0, y1);
+                } else if (!bounded || kmin < QPSO_EMPTY) {
+                    for (int32_t y = y0; y < y1; ++y)
+                        for (int32_t x = x0; x < x1; ++x)
+                            if (!(x >= dx0 && x < dx1 && y >= db0 && y < db1) && depth_at(fc, su, x, y) < QPSO_EMPTY)
+                                atomicOr(&mask[(y - b0) * MW + ((x - tx0) >> 5)], 1u << ((x - tx0) & 31));
+                }
+            }
+            const bool one = bounded && kmin == kmax;
+            if (col_line && x0 == minX) {
+                for (int32_t y = y0; y < y1; ++y) atomicMin(&col_d[y - b0], one ? kmin : depth_at(fc, su, minX, y));
+            }
+            if (row_line && y0 <= maxY && maxY < y1) {
+                for (int32_t x = x0; x < x1; ++x) atomicMin(&row_d[x - tx0], one ? kmin : depth_at(fc, su, x, maxY));
+            }
+            if (!in_d) return;
+            const int32_t w = px1 - px0, n = w * (py1 - py0);
+            int32_t qb;
             if (n > big_face && (qb = atomicAdd(&s_nqb, 1)) < kBQCap) {
-                bq_x[qb] = x0 | (x1 << 16);
-                bq_y[qb] = y0 | (y1 << 16);
+                bq_x[qb] = px0 | (px1 << 16);
+                bq_y[qb] = py0 | (py1 << 16);
                 bq_k[qb] = kd;
                 bq_i[qb] = (i << 1) | (flat ? 1 : 0);
             } else if (flat) {
-                const int32_t awrap = TW - w + 1;
-                int32_t* addr = tile + (y0 - b0) * TW + (x0 - tx0);
+                const int32_t awrap = DW - w + 1;
+                int32_t* addr = tile + (py0 - db0) * DW + (px0 - dx0);
                 int32_t c = w;
                 for (int32_t m = 0; m < n; ++m) {
                     atomicMin(addr, kd);
@@ -505,7 +598,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             } else {
                 const int q = atomicAdd(&s_nq, 1);
                 if (q < kQCap) queue[q] = i;
-                else raster_exact(fc, su, x0, x1, y0, y1, b0);
+                else raster_exact(fc, su, px0, px1, py0, py1);
             }
         };
         int32_t* __restrict__ ring = rings + (tid >> 6) * kRing;
@@ -545,14 +638,14 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         }
         if (lane < cnt) draw(ring[(head + lane) & (kRing - 1)]);
         __syncthreads();
-        // queued small faces: per-pixel depth, one face per thread
+        // queued small faces: per-pixel depth over their D part, one face per thread
         const int nq = min(s_nq, kQCap);
         for (int q = tid; q < nq; q += BT) {
             Face fc;
             Setup su;
             load_face(load_ref(faces, queue[q]), fc);
             face_setup(fc, su);
-            raster_exact(fc, su, max(fc.minx, tx0), min(fc.maxx, tx1), max(fc.miny, b0), min(fc.maxy, b1), b0);
+            raster_exact(fc, su, max(fc.minx, dx0), min(fc.maxx, dx1), max(fc.miny, db0), min(fc.maxy, db1));
         }
         // queued big faces: one face per wave, taken in turn
         const int nqb = min(s_nqb, kBQCap);
@@ -561,40 +654,73 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             if ((tid & 63) == 0) q = atomicAdd(&s_take, 1);
             q = __shfl(q, 0);
             if (q >= nqb) break;
-            raster_wave(q, b0);
+            raster_wave(q);
         }
         __syncthreads();
         if (tid == 0) { s_nq = 0; s_nqb = 0; s_take = 0; }
 
-        if (S.dbg_frames) {
+        if (full) {  // D is the whole tile here
             int32_t* fr = S.dbg_frames + (size_t)p * kRows * kCols;
-            for (int i = tid; i < nt; i += BT) fr[(b0 + i / TW) * kCols + tx0 + i % TW] = tile[i];
+            for (int i = tid; i < nd; i += BT) fr[(b0 + i / TW) * kCols + tx0 + i % TW] = tile[i];
         }
 
-        // ---- counters over the band's tile pixels outside the cluster rect,
-        //      weighted by visits (order-free); nothing is observed there, so a
-        //      rendered pixel counts in n_rendered and n_tr.  The pixels inside
-        //      the rect are counted by the error scan below, once per visit.
-        const int32_t dq = BT / TW, dr = BT % TW;
-        int32_t cy = b0 + tid / TW, cx = tx0 + tid % TW;
-        for (int i = tid; i < nt; i += BT, cx += dr, cy += dq) {
-            if (cx >= tx1) { cx -= TW; ++cy; }
-            const int32_t y = cy, x = cx;
-            if (tile[i] == QPSO_EMPTY || (x >= rx0 && x < rx1 && y >= ry0 && y < ry1)) continue;
-            const uint32_t w = (uint32_t)vis.at(x - minX, y - minY);
-            n_rend += w;
-            n_tr += w;
+        // ---- counters of the band's tile pixels outside the cluster rect,
+        //      weighted by visits (order-free): nothing is observed there, so
+        //      a rendered pixel counts in n_rendered and n_tr.  Pixels of D
+        //      inside the rect are counted by the error scan below, once per
+        //      visit.  The rect contains D unless D is the whole tile.
+        if (full) {
+            const int32_t dq = BT / TW, dr = BT % TW;
+            int32_t cy = b0 + tid / TW, cx = tx0 + tid % TW;
+            for (int i = tid; i < nd; i += BT, cx += dr, cy += dq) {
+                if (cx >= tx1) { cx -= TW; ++cy; }
+                const int32_t y = cy, x = cx;
+                if (tile[i] == QPSO_EMPTY || (x >= rx0 && x < rx1 && y >= ry0 && y < ry1)) continue;
+                const uint32_t w = (uint32_t)vis.at(x - minX, y - minY);
+                n_rend += w;
+                n_tr += w;
+            }
+        } else {
+            // the mask outside D, one row per thread: every pixel but (minX, y)
+            // is visited once, except in the row maxY, where only offsets
+            // [1, T - H W) are; (minX, y) is visited vis.at(0, k) times
+            for (int32_t y = b0 + tid; y < b1; y += BT) {
+                const int32_t k = y - minY;
+                const uint32_t* row = mask + (y - b0) * MW;
+                int32_t ca = tx0, cb = tx1;  // counted columns [ca, cb) besides (minX, y)
+                if (k == H) cb = min(cb, minX + T - H * W);
+                ca = max(ca, minX + 1);
+                const bool row_d_in = y >= db0 && y < db1;
+                uint32_t c = 0;
+                for (int32_t w = 0; w < MW && ca < cb; ++w) {
+                    const int32_t xw = tx0 + 32 * w;  // column of bit 0
+                    auto span = [&](int32_t a, int32_t b2) -> uint32_t {  // bits for columns [a, b2)
+                        const int32_t lo = max(a - xw, 0), hi = min(b2 - xw, 32);
+                        if (lo >= hi) return 0u;
+                        return (hi - lo == 32) ? ~0u : (((1u << (hi - lo)) - 1u) << lo);
+                    };
+                    uint32_t m = row[w] & span(ca, cb);
+                    if (row_d_in) m &= ~span(dx0, dx1);
+                    c += (uint32_t)__builtin_popcount(m);
+                }
+                if (col_line && !(row_d_in && minX >= dx0 && minX < dx1)) {
+                    const int32_t c0 = minX - tx0;
+                    if ((row[c0 >> 5] >> (c0 & 31)) & 1u) c += (uint32_t)vis.at(0, k);
+                }
+                n_rend += c;
+                n_tr += c;
+            }
         }
 
         // ---- squared error: the only order-sensitive sum.  Slot t & 2047
         //      accumulates in increasing t (main.cu:693-717); contributions come
-        //      only from tile pixels inside the cluster rect, so scan the t whose
+        //      only from D pixels inside the cluster rect, so scan the t whose
         //      row lies in this band and whose pixel lies in that intersection.
         //      Each t is one visit, so the counters of those pixels are summed
         //      here too.
-        const int32_t qx0 = max(tx0, rx0), qx1 = min(tx1, rx1);
-        const int32_t qy0 = max(b0, ry0), qy1 = min(b1, ry1);
-        if (qx0 < qx1 && qy0 < qy1) {
+        const int32_t qx0 = max(dx0, rx0), qx1 = min(dx1, rx1);
+        const int32_t qy0 = max(db0, ry0), qy1 = min(db1, ry1);
+        if (DW > 0 && qx0 < qx1 && qy0 < qy1) {
             const int32_t k0 = qy0 - minY, k1 = qy1 - minY;
             const int32_t tlo = min(k0 * W + (qx0 - minX), T);
             const int32_t thi = min(k1 * W + 1, T);
@@ -608,7 +734,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
                     if (x >= W) x -= W;  // t = kW mapped to row k-1
                     const int32_t px = minX + x, py = minY + y;
                     if (px < qx0 || px >= qx1 || py < qy0 || py >= qy1) continue;
-                    const int32_t zr = tile[(py - b0) * TW + px - tx0];
+                    const int32_t zr = tile[(py - db0) * DW + px - dx0];
                     const int32_t zk = (int32_t)obs[(py - ry0) * rw + (px - rx0)];
                     if (zr != QPSO_EMPTY) {
                         ++n_rend;
@@ -626,19 +752,19 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         }
 
         // ---- tile pixels the scan never visits keep their depth
-        if (minX >= tx0 && minX < tx1) {  // column x-offset 0
+        if (col_line) {  // column x-offset 0
             for (int32_t py = b0 + tid; py < b1; py += BT) {
                 if (py > maxY) continue;
                 if (vis.at(0, py - minY) == 0) {
-                    const int32_t v = tile[(py - b0) * TW + minX - tx0];
+                    const int32_t v = col_d[py - b0];
                     if (v != QPSO_EMPTY) push_stale(minX, py, v);
                 }
             }
         }
-        if (maxY >= b0 && maxY < b1) {  // last row, tail beyond t = T
+        if (row_line) {  // last row, tail beyond t = T
             const int32_t xs = max(max(1, T - H * W), tx0 - minX), xe = min(W, tx1 - minX);
             for (int32_t xb = xs + tid; xb < xe; xb += BT) {
-                const int32_t v = tile[(maxY - b0) * TW + minX + xb - tx0];
+                const int32_t v = row_d[minX + xb - tx0];
                 if (v != QPSO_EMPTY) push_stale(minX + xb, maxY, v);
             }
         }

@@ -330,8 +330,9 @@ QD void edge_range32(int32_t CY, int32_t FDX, int32_t FDY, int32_t ia, int32_t i
     hi = (e + max(dx, 0) + max(dy, 0) + 15) >> 8;
 }
 
-// Exact test for a face whose written depth is one integer k over the pixel
-// rectangle [x0, x1) x [y0, y1) of its bbox.  The edge value at a pixel is
+// Exact bounds [kmin, kmax] of the depth a face writes over the pixel
+// rectangle [x0, x1) x [y0, y1) of its bbox (false where they cannot be had
+// this way: a wrapping edge value or a NaN corner).  The edge value at a pixel is
 // CY + FDX (y - miny) - FDY (x - minx) mod 2^32; if that affine function stays
 // inside [INT32_MIN, INT32_MAX - 15] at the four corners it does so everywhere,
 // no wrap occurs, and cx = (CX + 15) >> 8 ranges between its corner values.
@@ -339,8 +340,8 @@ QD void edge_range32(int32_t CY, int32_t FDX, int32_t FDY, int32_t ia, int32_t i
 // z2, z3) and in the addend, and rint is monotone, so the corner combinations
 // bound every pixel's written value exactly.  The rectangle lies inside the
 // bbox, so the offsets are >= 0.
-QD bool flat_depth(const Face& fc, const Setup& s, int32_t x0, int32_t x1, int32_t y0, int32_t y1,
-                   int32_t& k) {
+QD bool depth_bounds(const Face& fc, const Setup& s, int32_t x0, int32_t x1, int32_t y0, int32_t y1,
+                     int32_t& kmin, int32_t& kmax) {
     const int32_t ia = x0 - fc.minx, ib = x1 - 1 - fc.minx, ja = y0 - fc.miny, jb = y1 - 1 - fc.miny;
     const uint32_t steps = (uint32_t)__builtin_abs(s.FDX23) | (uint32_t)__builtin_abs(s.FDY23) |
                            (uint32_t)__builtin_abs(s.FDX31) | (uint32_t)__builtin_abs(s.FDY31);
@@ -357,8 +358,22 @@ QD bool flat_depth(const Face& fc, const Setup& s, int32_t x0, int32_t x1, int32
     const float d0 = __builtin_fmaf((float)lo3, s.z3, fa), d1 = __builtin_fmaf((float)lo3, s.z3, fb);
     const float d2 = __builtin_fmaf((float)hi3, s.z3, fa), d3 = __builtin_fmaf((float)hi3, s.z3, fb);
     if (d0 != d0 || d1 != d1 || d2 != d2 || d3 != d3) return false;
-    k = rint_sat(__builtin_fminf(__builtin_fminf(d0, d1), __builtin_fminf(d2, d3)));
-    return k == rint_sat(__builtin_fmaxf(__builtin_fmaxf(d0, d1), __builtin_fmaxf(d2, d3)));
+    kmin = rint_sat(__builtin_fminf(__builtin_fminf(d0, d1), __builtin_fminf(d2, d3)));
+    kmax = rint_sat(__builtin_fmaxf(__builtin_fmaxf(d0, d1), __builtin_fmaxf(d2, d3)));
+    return true;
+}
+
+// The rectangle's written depth is the one integer k.
+QD bool flat_depth(const Face& fc, const Setup& s, int32_t x0, int32_t x1, int32_t y0, int32_t y1, int32_t& k) {
+    int32_t kmax;
+    return depth_bounds(fc, s, x0, x1, y0, y1, k, kmax) && k == kmax;
+}
+
+// Written depth at pixel (x, y) of the face's bbox, from the edge values there.
+QD int32_t depth_at(const Face& fc, const Setup& s, int32_t x, int32_t y) {
+    const int32_t CX2 = wsub(wadd(s.CY2, wmul(y - fc.miny, s.FDX23)), wmul(x - fc.minx, s.FDY23));
+    const int32_t CX3 = wsub(wadd(s.CY3, wmul(y - fc.miny, s.FDX31)), wmul(x - fc.minx, s.FDY31));
+    return pixel_depth(CX2, CX3, s);
 }
 
 // Fitness row index y(t) = trunc(f32(t) * rcp(f32(W))) (main.cu:645).

@@ -69,16 +69,20 @@ def oracle_eval(scene, params, poses):
     return np.array(fits, np.float32), np.array(aabbs, np.int32), np.array(frames)
 
 
-def check_eval(ctx, scene, params, poses):
+def check_eval(ctx, scene, params, poses, frames=True):
+    """Frames requested: the kernel keeps exact depth over its whole tile and
+    dumps it.  Without: exact depth only inside the cluster rect and the two
+    unvisited lines, a rendered-mask elsewhere (the solve path)."""
     mid = ctx.upload_mesh(scene.mesh_m())
     try:
         with ctx.solve(mid, scene.cluster(), params) as s:
-            fit, aabb, frames = s.eval_poses(poses, frames=True)
+            fit, aabb, fr = s.eval_poses(poses, frames=frames)
     finally:
         ctx.release_mesh(mid)
     ofit, oaabb, oframes = oracle_eval(scene, params, poses)
     assert_same_i32(aabb, oaabb, "aabb")
-    assert_same_i32(frames, oframes, "rendered depth buffers")
+    if frames:
+        assert_same_i32(fr, oframes, "rendered depth buffers")
     assert_same_f32(fit, ofit, "fitness")
     return fit
 
@@ -95,6 +99,14 @@ def test_eval_poses_config1_bands(ctx, scene1):
     scene, params = scene1
     poses = sample_poses(scene, params.n_particles, seed=2)
     check_eval(ctx, scene, params, poses)
+
+
+@pytest.mark.parametrize("wild", [0.0, 1.0])
+def test_eval_poses_mask_path(ctx, scene0, scene1, wild):
+    # far-off poses render mostly outside the cluster rect: mask, lines, bands
+    for k, (scene, params) in enumerate((scene0, scene1)):
+        poses = sample_poses(scene, params.n_particles, seed=11 + k, wild=wild)
+        check_eval(ctx, scene, params, poses, frames=False)
 
 
 def run_solve_pair(ctx, scene, params, threads=8):
@@ -145,8 +157,9 @@ def test_tile_configs_agree(scene1, cfg):
     os.environ["QPSO_RF_CFG"] = str(cfg)
     try:
         with qpso.Context(0) as c:
-            poses = sample_poses(scene, params.n_particles, seed=3 + cfg)
+            poses = sample_poses(scene, params.n_particles, seed=3 + cfg, wild=0.5)
             check_eval(c, scene, params, poses)
+            check_eval(c, scene, params, poses, frames=False)
     finally:
         if old is None:
             del os.environ["QPSO_RF_CFG"]
