@@ -550,16 +550,13 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             const int32_t px0 = max(x0, dx0), px1 = min(x1, dx1), py0 = max(y0, db0), py1 = min(y1, db1);
             const bool in_d = px0 < px1 && py0 < py1;
             const bool whole = in_d && px0 == x0 && px1 == x1 && py0 == y0 && py1 == y1;
-            int32_t kd = 0, kmin = 0, kmax = 0;
-            const bool flat = in_d && flat_depth(fc, su, px0, px1, py0, py1, kd);
-            // bounds over all of C, for the mask and the lines
-            bool bounded;
-            if (whole) {
-                bounded = flat;
-                kmin = kmax = kd;
-            } else {
-                bounded = depth_bounds(fc, su, x0, x1, y0, y1, kmin, kmax);
-            }
+            // bounds over all of C (for the mask and the lines); one depth over
+            // C is one over its D part, else the D part is tested by itself
+            int32_t kmin = 0, kmax = 0, kd = 0;
+            const bool bounded = depth_bounds(fc, su, x0, x1, y0, y1, kmin, kmax);
+            bool flat = in_d && bounded && kmin == kmax;
+            if (flat) kd = kmin;
+            else if (in_d && !whole) flat = flat_depth(fc, su, px0, px1, py0, py1, kd);
             if (MW && !whole) {
                 if (bounded && kmax < QPSO_EMPTY) {
                     mark_rows(x0, x1, y0, y1);
