@@ -583,14 +583,21 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
                 bq_k[qb] = kd;
                 bq_i[qb] = (i << 1) | (flat ? 1 : 0);
             } else if (flat) {
-                const int32_t awrap = DW - w + 1;
-                int32_t* addr = tile + (py0 - db0) * DW + (px0 - dx0);
-                int32_t c = w;
-                for (int32_t m = 0; m < n; ++m) {
-                    atomicMin(addr, kd);
-                    const bool wrap = --c == 0;
-                    addr += wrap ? awrap : 1;
-                    c = wrap ? w : c;
+                // rows in chunks of four pixels; a chunk's positions past the
+                // row's end repeat its last pixel (the same minimum again)
+                int32_t* row = tile + (py0 - db0) * DW + (px0 - dx0);
+                const int32_t last = w - 1;
+                int32_t c = 0, rows = py1 - py0;
+                while (rows > 0) {
+                    atomicMin(row + c, kd);
+                    atomicMin(row + min(c + 1, last), kd);
+                    atomicMin(row + min(c + 2, last), kd);
+                    atomicMin(row + min(c + 3, last), kd);
+                    c += 4;
+                    const bool next = c > last;
+                    row += next ? DW : 0;
+                    rows -= next ? 1 : 0;
+                    c = next ? 0 : c;
                 }
             } else {
                 const int q = atomicAdd(&s_nq, 1);
