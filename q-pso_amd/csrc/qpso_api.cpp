@@ -73,6 +73,8 @@ struct Mesh {
     QpsoFaceRef* faces = nullptr;  // interleaved raster order [F]
     uint32_t F = 0, V = 0;
     std::vector<uint32_t> v_prefix;  // unique vertices referenced by faces [0, f+1)
+    int32_t* runs = nullptr;         // (first face, first vertex, end vertex) per run
+    int n_runs = 0;
 };
 
 // Deduplicate the corners of the non-indexed SoA mesh (bitwise-equal
@@ -80,8 +82,37 @@ struct Mesh {
 // vertices of faces [0, f) are the prefix [0, v_prefix[f-1])) and build the
 // raster table in face order: neighbouring faces in one wave write
 // neighbouring pixels, which keeps ds_min_i32 free of bank conflicts.
+// Runs of consecutive faces whose corners span at most kRunVerts vertex
+// indices (the smallest projected-vertex cache of the kernels), greedy in face
+// order.  Vertices are numbered by first use, so neighbouring faces of an
+// ordinary mesh share a short index range.
+constexpr int32_t kRunVerts = 10112 / 6;
+void build_runs(const std::vector<int32_t>& tri, uint32_t F, std::vector<int32_t>& runs) {
+    runs.clear();
+    uint32_t f = 0;
+    while (f < F) {
+        const uint32_t start = f;
+        int32_t lo = INT32_MAX, hi = INT32_MIN;
+        for (; f < F; ++f) {
+            const int32_t* t = &tri[(size_t)f * 3];
+            const int32_t nlo = std::min(lo, std::min(t[0], std::min(t[1], t[2])));
+            const int32_t nhi = std::max(hi, std::max(t[0], std::max(t[1], t[2])));
+            if (f > start && nhi - nlo + 1 > kRunVerts) break;
+            lo = nlo;
+            hi = nhi;
+        }
+        if (hi - lo + 1 > kRunVerts) {  // one face alone spans more: no runs for this mesh
+            runs.clear();
+            return;
+        }
+        runs.push_back((int32_t)start);
+        runs.push_back(lo);
+        runs.push_back(hi + 1);
+    }
+}
+
 void build_mesh_tables(const float* A, uint32_t F, std::vector<float>& verts, std::vector<QpsoFaceRef>& order,
-                       std::vector<uint32_t>& v_prefix) {
+                       std::vector<uint32_t>& v_prefix, std::vector<int32_t>& runs) {
     struct Key {
         uint32_t x, y, z;
         bool operator==(const Key& o) const { return x == o.x && y == o.y && z == o.z; }
@@ -127,6 +158,7 @@ void build_mesh_tables(const float* A, uint32_t F, std::vector<float>& verts, st
     order.reserve(F);
     for (uint32_t f = 0; f < F; ++f)
         order.push_back(QpsoFaceRef{tri[f * 3], tri[f * 3 + 1], tri[f * 3 + 2], (int32_t)f});
+    build_runs(tri, F, runs);
 }
 
 struct ArenaLayout {
@@ -272,13 +304,20 @@ int read_result(qpso_solve* s, qpso_result* out, hipStream_t st) {
                                                : QPSO_OK;
 }
 
+bool needs_windows(const qpso_solve* s, int cfg) { return qpso_launch::rf_needs_windows(cfg, s->h.V_eff); }
+bool any_needs_windows(qpso_solve* const* solves, int n, int cfg) {
+    for (int i = 0; i < n; ++i)
+        if (needs_windows(solves[i], cfg)) return true;
+    return false;
+}
+
 // Launch the whole iteration loop of one solve (plain form).
 int launch_solve(qpso_solve* s, hipStream_t st) {
     qpso_ctx* ctx = s->ctx;
     HIP_TRY(ctx, qpso_launch::init(s->d_desc, 1, st));
     const QpsoItem* items = reinterpret_cast<const QpsoItem*>(s->arena + s->L.items);
     for (int it = 0; it < s->h.I; ++it) {
-        HIP_TRY(ctx, qpso_launch::render_fitness(s->rf_cfg, s->d_desc, items, s->h.N, st));
+        HIP_TRY(ctx, qpso_launch::render_fitness(s->rf_cfg, needs_windows(s, s->rf_cfg), s->d_desc, items, s->h.N, st));
         HIP_TRY(ctx, qpso_launch::pso_step(s->d_desc, 1, st));
     }
     return QPSO_OK;
@@ -374,6 +413,7 @@ int qpso_destroy(qpso_ctx* ctx) {
         hipFree(kv.second.d);
         hipFree(kv.second.verts);
         hipFree(kv.second.faces);
+        hipFree(kv.second.runs);
     }
     for (auto& kv : ctx->rand_cache) hipFree(kv.second);
     hipStreamDestroy(ctx->stream);
@@ -391,7 +431,9 @@ int qpso_mesh_upload(qpso_ctx* ctx, const float* soa, uint32_t n_faces, uint32_t
     m.F = n_faces;
     std::vector<float> verts;
     std::vector<QpsoFaceRef> order;
-    build_mesh_tables(soa, n_faces, verts, order, m.v_prefix);
+    std::vector<int32_t> runs;
+    build_mesh_tables(soa, n_faces, verts, order, m.v_prefix, runs);
+    m.n_runs = (int)(runs.size() / 3);
     m.V = (uint32_t)(verts.size() / 3);
     HIP_TRY(ctx, hipMalloc(&m.d, sizeof(float) * 9 * (size_t)n_faces));
     HIP_TRY(ctx, hipMalloc(&m.verts, sizeof(float) * verts.size()));
@@ -399,6 +441,8 @@ int qpso_mesh_upload(qpso_ctx* ctx, const float* soa, uint32_t n_faces, uint32_t
     HIP_TRY(ctx, hipMemcpy(m.d, soa, sizeof(float) * 9 * (size_t)n_faces, hipMemcpyHostToDevice));
     HIP_TRY(ctx, hipMemcpy(m.verts, verts.data(), sizeof(float) * verts.size(), hipMemcpyHostToDevice));
     HIP_TRY(ctx, hipMemcpy(m.faces, order.data(), sizeof(QpsoFaceRef) * order.size(), hipMemcpyHostToDevice));
+    HIP_TRY(ctx, hipMalloc(&m.runs, sizeof(int32_t) * runs.size()));
+    HIP_TRY(ctx, hipMemcpy(m.runs, runs.data(), sizeof(int32_t) * runs.size(), hipMemcpyHostToDevice));
     *mesh_id = ctx->next_mesh++;
     ctx->meshes[*mesh_id] = std::move(m);
     return QPSO_OK;
@@ -413,6 +457,7 @@ int qpso_mesh_release(qpso_ctx* ctx, uint32_t mesh_id) {
     hipFree(it->second.d);
     hipFree(it->second.verts);
     hipFree(it->second.faces);
+    hipFree(it->second.runs);
     ctx->meshes.erase(it);
     return QPSO_OK;
 }
@@ -457,6 +502,8 @@ int qpso_solve_create(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl, c
     h.V_eff = (int32_t)mit->second.v_prefix[h.F_eff - 1];
     h.faces = mit->second.faces;
     h.n_faces = (int32_t)mit->second.F;
+    h.runs = mit->second.runs;
+    h.n_runs = mit->second.n_runs;
     h.fx = p->cam.fx; h.fy = p->cam.fy; h.cx = p->cam.cx; h.cy = p->cam.cy;
     h.N = N; h.I = I; h.S = S;
     h.w = p->w; h.c1 = p->c1; h.c2 = p->c2;
@@ -556,7 +603,7 @@ int qpso_solve_run_timed(qpso_solve* s, qpso_result* out, qpso_timing* tm) {
         if (e == hipSuccess) e = qpso_launch::init(s->d_desc, 1, st);
         if (e == hipSuccess) e = hipEventRecord(ev[1], st);
         for (int it = 0; it < I && e == hipSuccess; ++it) {
-            e = qpso_launch::render_fitness(s->rf_cfg, s->d_desc, items, s->h.N, st);
+            e = qpso_launch::render_fitness(s->rf_cfg, needs_windows(s, s->rf_cfg), s->d_desc, items, s->h.N, st);
             if (e == hipSuccess) e = hipEventRecord(ev[2 + 2 * it], st);
             if (e == hipSuccess) e = qpso_launch::pso_step(s->d_desc, 1, st);
             if (e == hipSuccess) e = hipEventRecord(ev[3 + 2 * it], st);
@@ -666,7 +713,7 @@ int qpso_solve_eval_poses(qpso_solve* s, int n, const float* poses, float* fit, 
     tmp.dbg_aabb = d_aabb;
     if (hipMalloc(&d_tmp, sizeof(QpsoSolveDev)) != hipSuccess) { cleanup(); return QPSO_E_NOMEM; }
     hipMemcpyAsync(d_tmp, &tmp, sizeof(tmp), hipMemcpyHostToDevice, st);
-    hipError_t e = qpso_launch::render_fitness(s->rf_cfg, d_tmp,
+    hipError_t e = qpso_launch::render_fitness(s->rf_cfg, needs_windows(s, s->rf_cfg), d_tmp,
                                                reinterpret_cast<const QpsoItem*>(A + s->L.items), n, st);
     if (e == hipSuccess && fit) e = hipMemcpyAsync(fit, A + s->L.fit, sizeof(float) * n, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess && aabb) e = hipMemcpyAsync(aabb, d_aabb, sizeof(int32_t) * 4 * n, hipMemcpyDeviceToHost, st);
@@ -720,7 +767,7 @@ int qpso_autotune(qpso_ctx* ctx, int n, qpso_solve* const* solves, int iters, qp
     auto run = [&](int cfg) {
         hipError_t r = qpso_launch::init(d_descs, n, st);
         for (int it = 0; it < iters && r == hipSuccess; ++it) {
-            r = qpso_launch::render_fitness(cfg, d_descs, d_items, (int)items.size(), st);
+            r = qpso_launch::render_fitness(cfg, any_needs_windows(solves, n, cfg), d_descs, d_items, (int)items.size(), st);
             if (r == hipSuccess) r = qpso_launch::pso_step(d_descs, n, st);
         }
         return r;
@@ -801,7 +848,7 @@ int qpso_solve_run_batch(qpso_ctx* ctx, int n, qpso_solve* const* solves, qpso_r
         HIP_TRY(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         hipError_t e = qpso_launch::init(P.d_descs, n, st);
         for (int it = 0; it < P.max_iters && e == hipSuccess; ++it) {
-            e = qpso_launch::render_fitness(cfg, P.d_descs, P.d_items, P.n_items, st);
+            e = qpso_launch::render_fitness(cfg, any_needs_windows(solves, n, cfg), P.d_descs, P.d_items, P.n_items, st);
             if (e == hipSuccess) e = qpso_launch::pso_step(P.d_descs, n, st);
         }
         hipError_t e2 = hipStreamEndCapture(st, &graph);
@@ -881,7 +928,7 @@ void server_loop(qpso_server* sv) {
     hipEvent_t ev[kInFlight];
     for (auto& e : ev) hipEventCreateWithFlags(&e, hipEventDisableTiming);
     uint64_t launched = 0;
-    bool changed = true;
+    bool changed = true, win = false;
     for (;;) {
         std::vector<qpso_server::Req> admit;
         {
@@ -927,7 +974,9 @@ void server_loop(qpso_server* sv) {
         if (changed) {  // the copies are stream-ordered behind the steps that read the old arrays
             descs.clear();
             items.clear();
+            win = false;
             for (size_t i = 0; i < sv->active.size(); ++i) {
+                win |= needs_windows(sv->active[i].s, ctx->rf_cfg);
                 descs.push_back(sv->active[i].s->h);
                 for (int q = 0; q < sv->active[i].s->h.N; ++q) items.push_back(QpsoItem{(int32_t)i, q});
             }
@@ -949,7 +998,7 @@ void server_loop(qpso_server* sv) {
             changed = false;
         }
         if (launched >= (uint64_t)kInFlight) hipEventSynchronize(ev[launched % kInFlight]);
-        qpso_launch::render_fitness(ctx->rf_cfg, d_descs, d_items, (int)items.size(), sv->st);
+        qpso_launch::render_fitness(ctx->rf_cfg, win, d_descs, d_items, (int)items.size(), sv->st);
         qpso_launch::pso_step(d_descs, (int)descs.size(), sv->st);
         hipEventRecord(ev[launched % kInFlight], sv->st);
         ++launched;

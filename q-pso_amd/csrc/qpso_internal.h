@@ -41,6 +41,11 @@ struct QpsoSolveDev {
     int32_t V_eff;   // unique vertices referenced by faces < F_eff (a prefix)
     const QpsoFaceRef* faces;
     int32_t n_faces; // entries in `faces` (= F)
+    // runs of consecutive faces whose corners lie in a short vertex range:
+    // (first face, first vertex, end vertex) per run, for meshes whose
+    // vertices do not fit the projected-vertex cache at once
+    const int32_t* runs;
+    int32_t n_runs;
     // camera
     double fx, fy, cx, cy;
     // PSO

@@ -15,8 +15,12 @@ int rf_config_tile(int cfg);
 
 // InitQPSO for every solve in d_solves[0..n).
 hipError_t init(const QpsoSolveDev* d_solves, int n_solves, hipStream_t st);
+// Whether a mesh of v_eff vertices needs the window-by-window instantiation.
+bool rf_needs_windows(int cfg, int v_eff);
 // One fused render+fitness pass; d_items[i] = (solve index, particle index).
-hipError_t render_fitness(int cfg, const QpsoSolveDev* d_solves, const QpsoItem* d_items, int n_items,
+// win: the instantiation that draws meshes beyond the vertex cache window by
+// window (it handles every mesh; the other one only cached ones).
+hipError_t render_fitness(int cfg, bool win, const QpsoSolveDev* d_solves, const QpsoItem* d_items, int n_items,
                           hipStream_t st);
 // argmin + pbest/gbest + particle update, one workgroup per solve.
 hipError_t pso_step(const QpsoSolveDev* d_solves, int n_solves, hipStream_t st);

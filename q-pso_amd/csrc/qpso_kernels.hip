@@ -140,7 +140,9 @@ struct Visits {
     }
 };
 
-template <int BT, int CAP>
+// WIN: the instantiation that can draw a mesh window by window (larger than
+// the vertex cache); the other one keeps the cached path free of it.
+template <int BT, int CAP, bool WIN>
 __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __restrict__ solves,
                                                       const QpsoItem* __restrict__ items) {
     constexpr int SPT = 2048 / BT;  // fitness slots per thread
@@ -185,18 +187,25 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     __syncthreads();
 
     // The projected unique vertices stay in LDS when they take at most half of
-    // it; otherwise every pass projects the three corners of each face again.
+    // it (kWin vertices).  A larger mesh is cut on the host into runs of
+    // faces whose corners lie in a window of at most kWin vertex indices;
+    // each band then projects the windows in turn.  Without usable runs every
+    // face's corners are projected where needed.
+    constexpr int kWin = CAP / 6;
     const int Ve = S.V_eff;
-    const bool cached = 3 * Ve <= CAP / 2;
+    const bool cached = Ve <= kWin;
+    const bool runs = WIN && !cached && S.n_runs > 0 && S.n_runs * 256 <= Fe;  // runs of 256 faces on average
+    const int VC = cached ? Ve : (runs ? kWin : 0);  // cache capacity in vertices
     int32_t* __restrict__ vX = lds;
-    int32_t* __restrict__ vY = lds + Ve;
-    float* __restrict__ vZ = reinterpret_cast<float*>(lds + 2 * Ve);
+    int32_t* __restrict__ vY = lds + VC;
+    float* __restrict__ vZ = reinterpret_cast<float*>(lds + 2 * VC);
+    int32_t vbase = 0;  // vertex index of cache entry 0
     constexpr int kQCap = CAP / 32;   // small faces needing the per-pixel depth
     // faces of more than S.big_face pixels, rasterized by a whole wave so one
     // large face does not hold its wave's other 63 lanes idle: clipped
     // rectangle (x0 | x1 << 16, y0 | y1 << 16), constant depth, face << 1 | flat
     constexpr int kBQCap = CAP / 128;
-    int32_t* __restrict__ queue = cached ? lds + 3 * Ve : lds;
+    int32_t* __restrict__ queue = lds + 3 * VC;
     int32_t* __restrict__ bq_x = queue + kQCap;
     int32_t* __restrict__ bq_y = bq_x + kBQCap;
     int32_t* __restrict__ bq_k = bq_y + kBQCap;
@@ -211,13 +220,15 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
 
     // ---- pass A: per-face bbox and AABB over every face (main.h:1340-1353,
     //      AABBreduction main.h:1190-1304, final reduction main.cu:423-546).
-    //      (m + 15) >> 4 is monotone for |m| <= 2^30, so with the vertex cache
-    //      the face-bbox extremes are those of the vertex coordinates.
+    //      (m + 15) >> 4 is monotone for |m| <= 2^30, so the face-bbox
+    //      extremes are those of the vertex coordinates.
     {
         int bx0 = QPSO_AABB_MIN_SENT, bx1 = QPSO_AABB_MAX_SENT;
         int by0 = QPSO_AABB_MIN_SENT, by1 = QPSO_AABB_MAX_SENT;
         int dg = 0;
-        if (cached) {
+        {
+            // faces [0, Fe) use exactly the vertices [0, Ve); their bbox
+            // extremes are those of the vertices
             const QG float* __restrict__ wx = (const QG float*)S.verts;
             const QG float* __restrict__ wy = wx + S.V;
             const QG float* __restrict__ wz = wy + S.V;
@@ -226,21 +237,13 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
                 int32_t X, Y;
                 float Z;
                 dg |= vertex_fixed(pose, wx[v], wy[v], wz[v], cam, X, Y, Z) ? 1 : 0;
-                vX[v] = X; vY[v] = Y; vZ[v] = Z;
+                if (cached) { vX[v] = X; vY[v] = Y; vZ[v] = Z; }
                 xl = min(xl, X); xh = max(xh, X);
                 yl = min(yl, Y); yh = max(yh, Y);
             }
             if (xl <= xh) {
                 bx0 = min(bx0, wadd(xl, 15) >> 4); bx1 = max(bx1, wadd(xh, 15) >> 4);
                 by0 = min(by0, wadd(yl, 15) >> 4); by1 = max(by1, wadd(yh, 15) >> 4);
-            }
-        } else {
-            for (int f = tid; f < Fe; f += BT) {
-                Face fc;
-                face_project(mesh, F, f, pose, cam, fc);
-                bx0 = min(bx0, fc.minx); bx1 = max(bx1, fc.maxx);
-                by0 = min(by0, fc.miny); by1 = max(by1, fc.maxy);
-                dg |= fc.degen ? 1 : 0;
             }
         }
         bx0 = wave_min(bx0); bx1 = wave_max(bx1);
@@ -346,10 +349,12 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     };
     // the face's 28.4 vertices: from the LDS vertex cache, or projected again
     auto load_face = [&](const QpsoFaceRef& e, Face& fc) {
-        if (cached) {
-            fc.X[0] = vX[e.v0]; fc.Y[0] = vY[e.v0]; fc.Z[0] = vZ[e.v0];
-            fc.X[1] = vX[e.v1]; fc.Y[1] = vY[e.v1]; fc.Z[1] = vZ[e.v1];
-            fc.X[2] = vX[e.v2]; fc.Y[2] = vY[e.v2]; fc.Z[2] = vZ[e.v2];
+        if (VC > 0) {
+            const int32_t vb = WIN ? vbase : 0;
+            const int32_t a = e.v0 - vb, b = e.v1 - vb, c = e.v2 - vb;
+            fc.X[0] = vX[a]; fc.Y[0] = vY[a]; fc.Z[0] = vZ[a];
+            fc.X[1] = vX[b]; fc.Y[1] = vY[b]; fc.Z[1] = vZ[b];
+            fc.X[2] = vX[c]; fc.Y[2] = vY[c]; fc.Z[2] = vZ[c];
             face_bbox(fc);
         } else {
             face_project(mesh, F, e.face, pose, cam, fc);
@@ -479,7 +484,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     int32_t* col_d = nullptr;    // [band rows]: depth of (minX, y)
     int32_t* row_d = nullptr;    // [TW]: depth of (x, maxY)
     uint64_t fmask = ~0ull;  // bit k: face entry tid + k*BT still reaches later bands
-    const bool use_mask = n_faces <= 64 * BT;
+    const bool use_mask = !runs && n_faces <= 64 * BT;
     for (int32_t b0 = ty0, b1; has_tile && b0 < ty1; b0 = b1) {
         b1 = band_end(b0);  // band rows [b0, b1)
         const int32_t nb = b1 - b0;
@@ -608,16 +613,44 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         int32_t* __restrict__ ring = rings + (tid >> 6) * kRing;
         const int lane = tid & 63;
         const uint64_t below = (1ull << lane) - 1ull;
+        int run = 0;  // next run of faces (windowed meshes)
+        for (;;) {
+        int f_lo = 0, f_hi = n_faces;
+        if (runs) {
+            // the runs whose joint vertex window fits the cache; project it
+            const int32_t* __restrict__ rt = S.runs;
+            int32_t vlo = rt[3 * run + 1], vhi = rt[3 * run + 2];
+            f_lo = rt[3 * run];
+            int r2 = run + 1;
+            while (r2 < S.n_runs && rt[3 * r2] < Fe && max(vhi, rt[3 * r2 + 2]) - min(vlo, rt[3 * r2 + 1]) <= kWin) {
+                vlo = min(vlo, rt[3 * r2 + 1]);
+                vhi = max(vhi, rt[3 * r2 + 2]);
+                ++r2;
+            }
+            f_hi = r2 < S.n_runs ? rt[3 * r2] : n_faces;
+            run = r2;
+            vbase = vlo;
+            const QG float* __restrict__ wx = (const QG float*)S.verts;
+            const QG float* __restrict__ wy = wx + S.V;
+            const QG float* __restrict__ wz = wy + S.V;
+            for (int v = vlo + tid; v < vhi; v += BT) {
+                int32_t X, Y;
+                float Z;
+                vertex_fixed(pose, wx[v], wy[v], wz[v], cam, X, Y, Z);
+                vX[v - vlo] = X; vY[v - vlo] = Y; vZ[v - vlo] = Z;
+            }
+            __syncthreads();
+        }
         int head = 0, cnt = 0;  // wave-uniform ring state
         int k = 0;
         QpsoFaceRef nxt{0, 0, 0, INT32_MAX};
-        if (tid < n_faces) nxt = load_ref(faces, tid);
-        for (int base = tid - lane; base < n_faces; base += BT, ++k) {
-            const int i = base + lane;  // = tid + k * BT
+        if (f_lo + tid < f_hi) nxt = load_ref(faces, f_lo + tid);
+        for (int base = f_lo + tid - lane; base < f_hi; base += BT, ++k) {
+            const int i = base + lane;  // = f_lo + tid + k * BT
             const QpsoFaceRef e = nxt;  // prefetched one iteration ahead
-            if (i + BT < n_faces) nxt = load_ref(faces, i + BT);
+            if (i + BT < f_hi) nxt = load_ref(faces, i + BT);
             bool keep = false;
-            if (i < n_faces && (!use_mask || ((fmask >> k) & 1ull))) {
+            if (i < f_hi && (!use_mask || ((fmask >> k) & 1ull))) {
                 bool later = false;
                 if (e.face < Fe) {
                     Face fc;
@@ -662,6 +695,9 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         }
         __syncthreads();
         if (tid == 0) { s_nq = 0; s_nqb = 0; s_take = 0; }
+        if (!runs || run >= S.n_runs || S.runs[3 * run] >= Fe) break;
+        __syncthreads();  // the counters are reset before the next window's faces
+        }
 
         if (full) {  // D is the whole tile here
             int32_t* fr = S.dbg_frames + (size_t)p * kRows * kCols;
@@ -985,22 +1021,20 @@ namespace qpso_launch {
 
 struct RfCfg {
     int bt, cap;
-    const void* fn;
 };
 
-#define QPSO_RF_CFG(BT, CAP) {BT, CAP, reinterpret_cast<const void*>(&k_render_fitness<BT, CAP>)}
 static const RfCfg kRfCfgs[] = {
-    QPSO_RF_CFG(1024, 40448),  // 1 workgroup / CU, 158 KiB tile
-    QPSO_RF_CFG(512, 40448),   // 1 workgroup / CU, 8 waves
-    QPSO_RF_CFG(512, 20224),   // 2 workgroups / CU
-    QPSO_RF_CFG(256, 20224),   // 2 workgroups / CU, 4 waves each
-    QPSO_RF_CFG(256, 10112),   // 4 workgroups / CU
+    {1024, 40448},  // 1 workgroup / CU, 158 KiB tile
+    {512, 40448},   // 1 workgroup / CU, 8 waves
+    {512, 20224},   // 2 workgroups / CU
+    {256, 20224},   // 2 workgroups / CU, 4 waves each
+    {256, 10112},   // 4 workgroups / CU
 };
-#undef QPSO_RF_CFG
 
 int rf_num_configs() { return (int)(sizeof(kRfCfgs) / sizeof(kRfCfgs[0])); }
 int rf_config_threads(int cfg) { return kRfCfgs[cfg].bt; }
 int rf_config_tile(int cfg) { return kRfCfgs[cfg].cap; }
+bool rf_needs_windows(int cfg, int v_eff) { return v_eff > kRfCfgs[cfg].cap / 6; }
 
 hipError_t init(const QpsoSolveDev* d_solves, int n_solves, hipStream_t st) {
     if (n_solves <= 0) return hipSuccess;
@@ -1008,15 +1042,21 @@ hipError_t init(const QpsoSolveDev* d_solves, int n_solves, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t render_fitness(int cfg, const QpsoSolveDev* d_solves, const QpsoItem* d_items, int n_items,
+template <int BT, int CAP>
+static void launch_rf(bool win, const QpsoSolveDev* d_solves, const QpsoItem* d_items, int n_items, hipStream_t st) {
+    if (win) hipLaunchKernelGGL((k_render_fitness<BT, CAP, true>), dim3(n_items), dim3(BT), 0, st, d_solves, d_items);
+    else hipLaunchKernelGGL((k_render_fitness<BT, CAP, false>), dim3(n_items), dim3(BT), 0, st, d_solves, d_items);
+}
+
+hipError_t render_fitness(int cfg, bool win, const QpsoSolveDev* d_solves, const QpsoItem* d_items, int n_items,
                           hipStream_t st) {
     if (n_items <= 0) return hipSuccess;
     switch (cfg) {
-        case 0: hipLaunchKernelGGL((k_render_fitness<1024, 40448>), dim3(n_items), dim3(1024), 0, st, d_solves, d_items); break;
-        case 1: hipLaunchKernelGGL((k_render_fitness<512, 40448>), dim3(n_items), dim3(512), 0, st, d_solves, d_items); break;
-        case 2: hipLaunchKernelGGL((k_render_fitness<512, 20224>), dim3(n_items), dim3(512), 0, st, d_solves, d_items); break;
-        case 3: hipLaunchKernelGGL((k_render_fitness<256, 20224>), dim3(n_items), dim3(256), 0, st, d_solves, d_items); break;
-        case 4: hipLaunchKernelGGL((k_render_fitness<256, 10112>), dim3(n_items), dim3(256), 0, st, d_solves, d_items); break;
+        case 0: launch_rf<1024, 40448>(win, d_solves, d_items, n_items, st); break;
+        case 1: launch_rf<512, 40448>(win, d_solves, d_items, n_items, st); break;
+        case 2: launch_rf<512, 20224>(win, d_solves, d_items, n_items, st); break;
+        case 3: launch_rf<256, 20224>(win, d_solves, d_items, n_items, st); break;
+        case 4: launch_rf<256, 10112>(win, d_solves, d_items, n_items, st); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

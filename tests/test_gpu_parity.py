@@ -149,6 +149,21 @@ def test_solve_config1_short(ctx, scene1):
     check_solve(*run_solve_pair(ctx, scene, params))
 
 
+def test_big_mesh_windows(ctx):
+    # 16k faces, 8.2k unique vertices: more than the LDS vertex cache holds,
+    # so the faces are drawn run by run from projected vertex windows
+    mesh = qpso.mesh_uv_ellipsoid(103, 81, 70.0, 50.0, 35.0, 4, 8.0, 16.0, 5)
+    rng = np.random.default_rng(77)
+    pose = W.gt_pose(rng, xy=60.0, z=(700.0, 900.0), seed=77)
+    depth, labels = qpso.scene_render([(mesh, pose)], W.CAM, 1100.0, 2.0, 0.05, 78)
+    scene = W.Scene([(mesh, pose)], depth, labels)
+    params = qpso.make_params(24, 3, w=0.7, c1=1.5, c2=1.5, cam=W.CAM, seed=0)
+    poses = sample_poses(scene, params.n_particles, seed=21, wild=0.3)
+    check_eval(ctx, scene, params, poses, frames=False)
+    check_eval(ctx, scene, params, poses)
+    check_solve(*run_solve_pair(ctx, scene, params))
+
+
 @pytest.mark.parametrize("cfg", [1, 2, 3, 4])
 def test_tile_configs_agree(scene1, cfg):
     # every (threads, tile) variant of the render+fitness kernel is bit-exact
