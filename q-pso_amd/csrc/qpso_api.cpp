@@ -33,7 +33,7 @@ namespace {
 constexpr int kRows = QPSO_ROWS;
 constexpr int kCols = QPSO_COLS;
 constexpr int kFrame = kRows * kCols;
-constexpr int kStaleCap = 1024;
+constexpr int kStaleCap = QPSO_STALE_CAP;
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
@@ -162,7 +162,7 @@ void build_mesh_tables(const float* A, uint32_t F, std::vector<float>& verts, st
 }
 
 struct ArenaLayout {
-    size_t desc, items, obs, pos, vel, pbest, pbest_fit, fit, gbest, iter, flags, counters;
+    size_t desc, obs, pos, vel, pbest, pbest_fit, fit, gbest, iter, flags, counters;
     size_t stale0, stale1, cnt0, cnt1, fit_tr, gbest_tr, argmin_tr, total;
 };
 
@@ -171,7 +171,6 @@ ArenaLayout layout(int N, int I, int rw, int rh, bool trace) {
     size_t o = 0;
     auto take = [&](size_t bytes) { size_t r = o; o = align_up(o + bytes, 256); return r; };
     L.desc = take(sizeof(QpsoSolveDev));
-    L.items = take(sizeof(QpsoItem) * N);
     L.obs = take(sizeof(uint16_t) * (size_t)rw * rh);
     L.pos = take(sizeof(float) * 7 * N);
     L.vel = take(sizeof(float) * 7 * N);
@@ -315,9 +314,8 @@ bool any_needs_windows(qpso_solve* const* solves, int n, int cfg) {
 int launch_solve(qpso_solve* s, hipStream_t st) {
     qpso_ctx* ctx = s->ctx;
     HIP_TRY(ctx, qpso_launch::init(s->d_desc, 1, st));
-    const QpsoItem* items = reinterpret_cast<const QpsoItem*>(s->arena + s->L.items);
     for (int it = 0; it < s->h.I; ++it) {
-        HIP_TRY(ctx, qpso_launch::render_fitness(s->rf_cfg, needs_windows(s, s->rf_cfg), s->d_desc, items, s->h.N, st));
+        HIP_TRY(ctx, qpso_launch::render_fitness(s->rf_cfg, needs_windows(s, s->rf_cfg), s->d_desc, nullptr, s->h.N, st));
         HIP_TRY(ctx, qpso_launch::pso_step(s->d_desc, 1, st));
     }
     return QPSO_OK;
@@ -545,12 +543,9 @@ int qpso_solve_create(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl, c
     for (int y = 0; y < rh; ++y)
         std::memcpy(&crop[(size_t)y * rw], cl->depth + (size_t)(cl->rect[1] + y) * kCols + cl->rect[0],
                     sizeof(uint16_t) * rw);
-    std::vector<QpsoItem> items(N);
-    for (int i = 0; i < N; ++i) items[i] = QpsoItem{0, i};
     hipStream_t st = ctx->stream;
     int rc = QPSO_OK;
     if (hipMemcpyAsync(A + s->L.obs, crop.data(), crop.size() * 2, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(A + s->L.items, items.data(), items.size() * sizeof(QpsoItem), hipMemcpyHostToDevice, st) != hipSuccess ||
         hipMemcpyAsync(s->d_desc, &s->h, sizeof(QpsoSolveDev), hipMemcpyHostToDevice, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
         rc = set_err(ctx, QPSO_E_HIP, "solve upload failed");
@@ -597,13 +592,12 @@ int qpso_solve_run_timed(qpso_solve* s, qpso_result* out, qpso_timing* tm) {
         for (auto& e : s->timed_ev) HIP_TRY(ctx, hipEventCreate(&e));
     }
     const std::vector<hipEvent_t>& ev = s->timed_ev;
-    const QpsoItem* items = reinterpret_cast<const QpsoItem*>(s->arena + s->L.items);
     auto enqueue = [&]() {
         hipError_t e = hipEventRecord(ev[0], st);
         if (e == hipSuccess) e = qpso_launch::init(s->d_desc, 1, st);
         if (e == hipSuccess) e = hipEventRecord(ev[1], st);
         for (int it = 0; it < I && e == hipSuccess; ++it) {
-            e = qpso_launch::render_fitness(s->rf_cfg, needs_windows(s, s->rf_cfg), s->d_desc, items, s->h.N, st);
+            e = qpso_launch::render_fitness(s->rf_cfg, needs_windows(s, s->rf_cfg), s->d_desc, nullptr, s->h.N, st);
             if (e == hipSuccess) e = hipEventRecord(ev[2 + 2 * it], st);
             if (e == hipSuccess) e = qpso_launch::pso_step(s->d_desc, 1, st);
             if (e == hipSuccess) e = hipEventRecord(ev[3 + 2 * it], st);
@@ -713,8 +707,7 @@ int qpso_solve_eval_poses(qpso_solve* s, int n, const float* poses, float* fit, 
     tmp.dbg_aabb = d_aabb;
     if (hipMalloc(&d_tmp, sizeof(QpsoSolveDev)) != hipSuccess) { cleanup(); return QPSO_E_NOMEM; }
     hipMemcpyAsync(d_tmp, &tmp, sizeof(tmp), hipMemcpyHostToDevice, st);
-    hipError_t e = qpso_launch::render_fitness(s->rf_cfg, needs_windows(s, s->rf_cfg), d_tmp,
-                                               reinterpret_cast<const QpsoItem*>(A + s->L.items), n, st);
+    hipError_t e = qpso_launch::render_fitness(s->rf_cfg, needs_windows(s, s->rf_cfg), d_tmp, nullptr, n, st);
     if (e == hipSuccess && fit) e = hipMemcpyAsync(fit, A + s->L.fit, sizeof(float) * n, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess && aabb) e = hipMemcpyAsync(aabb, d_aabb, sizeof(int32_t) * 4 * n, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess && frames)

@@ -15,6 +15,10 @@
 // flags[] bits
 #define QPSO_FLAG_STALE_OVERFLOW 1
 
+// carried-over pixels kept per particle and iteration; more set
+// QPSO_FLAG_STALE_OVERFLOW
+#define QPSO_STALE_CAP 1024
+
 struct QpsoStale {  // one carried-over depth-buffer pixel
     int32_t pos;    // (y << 16) | x
     int32_t val;

@@ -17,7 +17,8 @@ int rf_config_tile(int cfg);
 hipError_t init(const QpsoSolveDev* d_solves, int n_solves, hipStream_t st);
 // Whether a mesh of v_eff vertices needs the window-by-window instantiation.
 bool rf_needs_windows(int cfg, int v_eff);
-// One fused render+fitness pass; d_items[i] = (solve index, particle index).
+// One fused render+fitness pass; d_items[i] = (solve index, particle index),
+// or d_items = nullptr for workgroup i = particle i of d_solves[0].
 // win: the instantiation that draws meshes beyond the vertex cache window by
 // window (it handles every mesh; the other one only cached ones).
 hipError_t render_fitness(int cfg, bool win, const QpsoSolveDev* d_solves, const QpsoItem* d_items, int n_items,

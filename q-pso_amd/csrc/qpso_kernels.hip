@@ -158,16 +158,51 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     __shared__ unsigned long long s_px[2];
 
     const int tid = threadIdx.x;
-    const QpsoItem item = items[blockIdx.x];
+    // no item table: one solve, workgroup b renders particle b
+    const QpsoItem item = items ? items[blockIdx.x] : QpsoItem{0, (int32_t)blockIdx.x};
     const QpsoSolveDev& S = solves[item.solve];
     const int p = item.particle;
-    const int iter = *S.iter;
-    if (iter >= S.I) return;  // this solve has finished (batched runs)
-
+    // Start-up loads go out together through global (not flat) pointers, so
+    // no barrier waits on them: the iteration, the pose, both carried-pixel
+    // counts and the vertices of pass A (kVPT per thread, the whole vertex
+    // cache).  Then the carried pixels (at most QPSO_STALE_CAP, kSPT per
+    // thread, kept in registers for the whole pass).  All are in bounds
+    // whatever the iteration, so they load before the finished-solve test.
     const int N = S.N;
+    const int iter_v = ((const QG int32_t*)S.iter)[0];
     float pose[7];
 #pragma unroll
-    for (int d = 0; d < 7; ++d) pose[d] = S.pos[d * N + p];
+    for (int d = 0; d < 7; ++d) pose[d] = ((const QG float*)S.pos)[d * N + p];
+    const int n_old0 = ((const QG int32_t*)S.stale_cnt[0])[p];
+    const int n_old1 = ((const QG int32_t*)S.stale_cnt[1])[p];
+    constexpr int kWin = CAP / 6;  // vertex cache capacity
+    constexpr int kVPT = (kWin + BT - 1) / BT;
+    const int Ve = S.V_eff;
+    const QG float* __restrict__ wx = (const QG float*)S.verts;
+    const QG float* __restrict__ wy = wx + S.V;
+    const QG float* __restrict__ wz = wy + S.V;
+    float ux[kVPT], uy[kVPT], uz[kVPT];
+#pragma unroll
+    for (int j = 0; j < kVPT; ++j) {
+        const int v = min(tid + j * BT, Ve - 1);  // Ve >= 1
+        ux[j] = wx[v]; uy[j] = wy[v]; uz[j] = wz[v];
+    }
+    const int iter = __builtin_amdgcn_readfirstlane(iter_v);
+    const int par = iter & 1;
+    const int n_old = __builtin_amdgcn_readfirstlane(par ? n_old1 : n_old0);
+    const int cap = S.stale_cap;
+    constexpr int kSPT = (QPSO_STALE_CAP + BT - 1) / BT;
+    QpsoStale carried[kSPT];
+    {
+        const QG QpsoStale* __restrict__ old = (const QG QpsoStale*)(par ? S.stale[1] : S.stale[0]) + (size_t)p * cap;
+#pragma unroll
+        for (int j = 0; j < kSPT; ++j) {
+            const int k = tid + j * BT;
+            carried[j] = k < n_old ? load_stale(old, k) : QpsoStale{0, 0};
+        }
+    }
+    if (iter >= S.I) return;  // this solve has finished (batched runs)
+
     const Cam cam{S.fx, S.fy, S.cx, S.cy};
     const float* __restrict__ mesh = S.mesh;
     const int F = S.F, Fe = S.F_eff;
@@ -191,8 +226,6 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     // faces whose corners lie in a window of at most kWin vertex indices;
     // each band then projects the windows in turn.  Without usable runs every
     // face's corners are projected where needed.
-    constexpr int kWin = CAP / 6;
-    const int Ve = S.V_eff;
     const bool cached = Ve <= kWin;
     const bool runs = WIN && !cached && S.n_runs > 0 && S.n_runs * 256 <= Fe;  // runs of 256 faces on average
     const int VC = cached ? Ve : (runs ? kWin : 0);  // cache capacity in vertices
@@ -229,18 +262,21 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         {
             // faces [0, Fe) use exactly the vertices [0, Ve); their bbox
             // extremes are those of the vertices
-            const QG float* __restrict__ wx = (const QG float*)S.verts;
-            const QG float* __restrict__ wy = wx + S.V;
-            const QG float* __restrict__ wz = wy + S.V;
             int32_t xl = INT32_MAX, xh = INT32_MIN, yl = INT32_MAX, yh = INT32_MIN;
-            for (int v = tid; v < Ve; v += BT) {
+            auto take = [&](int v, float x, float y, float z) {
                 int32_t X, Y;
                 float Z;
-                dg |= vertex_fixed(pose, wx[v], wy[v], wz[v], cam, X, Y, Z) ? 1 : 0;
+                dg |= vertex_fixed(pose, x, y, z, cam, X, Y, Z) ? 1 : 0;
                 if (cached) { vX[v] = X; vY[v] = Y; vZ[v] = Z; }
                 xl = min(xl, X); xh = max(xh, X);
                 yl = min(yl, Y); yh = max(yh, Y);
+            };
+#pragma unroll
+            for (int j = 0; j < kVPT; ++j) {
+                const int v = tid + j * BT;
+                if (v < Ve) take(v, ux[j], uy[j], uz[j]);
             }
+            for (int v = tid + kVPT * BT; v < Ve; v += BT) take(v, wx[v], wy[v], wz[v]);  // beyond the cache
             if (xl <= xh) {
                 bx0 = min(bx0, wadd(xl, 15) >> 4); bx1 = max(bx1, wadd(xh, 15) >> 4);
                 by0 = min(by0, wadd(yl, 15) >> 4); by1 = max(by1, wadd(yh, 15) >> 4);
@@ -267,10 +303,6 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
 #pragma unroll
         for (int i = 0; i < 4; ++i) S.dbg_aabb[p * 4 + i] = s_box[i];
     }
-    const int par = iter & 1;
-    const int cap = S.stale_cap;
-    const QG QpsoStale* __restrict__ old = (const QG QpsoStale*)S.stale[par] + (size_t)p * cap;
-    const int n_old = S.stale_cnt[par][p];
     QG QpsoStale* __restrict__ nw = (QG QpsoStale*)S.stale[par ^ 1] + (size_t)p * cap;
 
     if (degen) {  // guard G: NaN fitness, buffer reset
@@ -293,8 +325,10 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     // carried pixels inside the union box join the tile region
     {
         int32_t sx0 = INT32_MAX, sx1 = INT32_MIN, sy0 = INT32_MAX, sy1 = INT32_MIN;
-        for (int k = tid; k < n_old; k += BT) {
-            const QpsoStale e = load_stale(old, k);
+#pragma unroll
+        for (int j = 0; j < kSPT; ++j) {
+            if (tid + j * BT >= n_old) break;
+            const QpsoStale e = carried[j];
             const int32_t ex = e.pos & 0xffff, ey = e.pos >> 16;
             if (ex >= minX && ex < maxX && ey >= minY && ey <= maxY) {
                 sx0 = min(sx0, ex); sx1 = max(sx1, ex);
@@ -506,8 +540,10 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         }
         __syncthreads();
         // pixels carried over from earlier iterations (their values are below 60000)
-        for (int k = tid; k < n_old; k += BT) {
-            const QpsoStale e = load_stale(old, k);
+#pragma unroll
+        for (int j = 0; j < kSPT; ++j) {
+            if (tid + j * BT >= n_old) break;
+            const QpsoStale e = carried[j];
             const int32_t ex = e.pos & 0xffff, ey = e.pos >> 16;
             if (ey >= b0 && ey < b1 && ex >= tx0 && ex < tx1) {
                 if (ex >= dx0 && ex < dx1 && ey >= db0 && ey < db1) tile[(ey - db0) * DW + ex - dx0] = e.val;
@@ -808,12 +844,16 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
                 if (v != QPSO_EMPTY) push_stale(minX + xb, maxY, v);
             }
         }
-        __syncthreads();
+        // the next band refills the tile; after the last one nothing reads
+        // or writes it
+        if (b1 < ty1) __syncthreads();
     }
 
     // carried pixels outside this iteration's union box stay as they are
-    for (int k = tid; k < n_old; k += BT) {
-        const QpsoStale e = load_stale(old, k);
+#pragma unroll
+    for (int j = 0; j < kSPT; ++j) {
+        if (tid + j * BT >= n_old) break;
+        const QpsoStale e = carried[j];
         const int32_t ex = e.pos & 0xffff, ey = e.pos >> 16;
         if (!(ex >= minX && ex < maxX && ey >= minY && ey <= maxY)) push_stale(ex, ey, e.val);
     }
@@ -852,16 +892,22 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         atomicAdd(&s_px[1], px_f);
     }
 
-    float* red = reinterpret_cast<float*>(tile);  // the tile is dead now
+    // the wave rings (2 BT words) are dead since the last draw; one barrier,
+    // then wave 0 runs the rest of the tree: slot t = lane + 64 k sits in
+    // v[k], so level "t += t + stride" is v[k] += v[k + stride / 64]
+    float* red = reinterpret_cast<float*>(rings);
     red[tid] = acc_e[0];
-#pragma unroll
-    for (int stride = BT / 2; stride >= 64; stride >>= 1) {
-        __syncthreads();
-        if (tid < stride) red[tid] = red[tid] + red[tid + stride];
-    }
     __syncthreads();
     if (tid < 64) {
-        float v = red[tid];
+        float r[BT / 64];
+#pragma unroll
+        for (int k = 0; k < BT / 64; ++k) r[k] = red[tid + 64 * k];
+#pragma unroll
+        for (int s = BT / 128; s >= 1; s >>= 1) {
+#pragma unroll
+            for (int k = 0; k < s; ++k) r[k] = r[k] + r[k + s];
+        }
+        float v = r[0];
 #pragma unroll
         for (int stride = 32; stride >= 1; stride >>= 1) {
             const float o = __shfl_down(v, stride);
