@@ -479,10 +479,19 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             const int32_t sw = x1 - x0, n = sw * (y1 - y0);
             const int32_t dq = BT / sw, dr = BT % sw;
             int32_t y = y0 + tid / sw, x = x0 + tid % sw;
-            for (int i = tid; i < n; i += BT) {
-                if (obs[(y - ry0) * rw + (x - rx0)] != 0) n_seg += (uint32_t)vis.at(x - minX, y - minY);
-                x += dr; y += dq;
-                if (x >= x1) { x -= sw; ++y; }
+            constexpr int U = 4;  // loads in flight per thread
+            for (int i = tid; i < n; i += U * BT) {
+                int32_t o[U], xs[U], ys[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    o[u] = i + u * BT < n ? (int32_t)obs[(y - ry0) * rw + (x - rx0)] : 0;
+                    xs[u] = x; ys[u] = y;
+                    x += dr; y += dq;
+                    if (x >= x1) { x -= sw; ++y; }
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (o[u] != 0) n_seg += (uint32_t)vis.at(xs[u] - minX, ys[u] - minY);
             }
         };
         if (has_tile) {
@@ -800,28 +809,44 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             const int32_t k0 = qy0 - minY, k1 = qy1 - minY;
             const int32_t tlo = min(k0 * W + (qx0 - minX), T);
             const int32_t thi = min(k1 * W + 1, T);
-            for (int32_t base = (tlo / 2048) * 2048; base < thi; base += 2048) {
+            // U steps of 2048 per batch: all their loads go out before the
+            // first sum (a pixel outside Q reads as empty and observed 0,
+            // which adds nothing), then each slot sums in increasing t
+            constexpr int U = SPT >= 8 ? 1 : 8 / SPT;
+            for (int32_t base = (tlo / 2048) * 2048; base < thi; base += 2048 * U) {
+                int32_t zr[U][SPT], zk[U][SPT];
 #pragma unroll
-                for (int j = 0; j < SPT; ++j) {
-                    const int32_t t = base + j * BT + tid;  // slot t & 2047 = j*BT + tid
-                    if (t < tlo || t >= thi) continue;
-                    const int32_t y = row_of(t, vis.rW);
-                    int32_t x = t - y * W;
-                    if (x >= W) x -= W;  // t = kW mapped to row k-1
-                    const int32_t px = minX + x, py = minY + y;
-                    if (px < qx0 || px >= qx1 || py < qy0 || py >= qy1) continue;
-                    const int32_t zr = tile[(py - db0) * DW + px - dx0];
-                    const int32_t zk = (int32_t)obs[(py - ry0) * rw + (px - rx0)];
-                    if (zr != QPSO_EMPTY) {
-                        ++n_rend;
-                        if (zk == 0) {
-                            ++n_tr;
-                        } else {
-                            const float e = (float)wsub(zr, zk) * 0.001f;
-                            acc_e[j] = __builtin_fmaf(e, e, acc_e[j]);
+                for (int u = 0; u < U; ++u) {
+#pragma unroll
+                    for (int j = 0; j < SPT; ++j) {
+                        const int32_t t = base + u * 2048 + j * BT + tid;  // slot t & 2047 = j*BT + tid
+                        zr[u][j] = QPSO_EMPTY;
+                        zk[u][j] = 0;
+                        if (t < tlo || t >= thi) continue;
+                        const int32_t y = row_of(t, vis.rW);
+                        int32_t x = t - y * W;
+                        if (x >= W) x -= W;  // t = kW mapped to row k-1
+                        const int32_t px = minX + x, py = minY + y;
+                        if (px < qx0 || px >= qx1 || py < qy0 || py >= qy1) continue;
+                        zr[u][j] = tile[(py - db0) * DW + px - dx0];
+                        zk[u][j] = (int32_t)obs[(py - ry0) * rw + (px - rx0)];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+#pragma unroll
+                    for (int j = 0; j < SPT; ++j) {
+                        if (zr[u][j] != QPSO_EMPTY) {
+                            ++n_rend;
+                            if (zk[u][j] == 0) {
+                                ++n_tr;
+                            } else {
+                                const float e = (float)wsub(zr[u][j], zk[u][j]) * 0.001f;
+                                acc_e[j] = __builtin_fmaf(e, e, acc_e[j]);
+                            }
+                        } else if (zk[u][j] != 0) {
+                            ++n_seg;
                         }
-                    } else if (zk != 0) {
-                        ++n_seg;
                     }
                 }
             }
@@ -938,20 +963,43 @@ __device__ __forceinline__ bool better(float av, int ai, float bv, int bi) {
     return av < bv || (av == bv && ai < bi);
 }
 
-__device__ void update_particle(const QpsoSolveDev& S, int p, int iter, const float* gb) {
+// The state one particle update reads, held in registers.
+struct Particle {
+    float P[7], V[7], B[7];  // pos, vel, pbest
+    float R[12];             // the iteration's random numbers, in order of use
+    float fit, pbest_fit;
+};
+
+__device__ __forceinline__ void load_particle(const QpsoSolveDev& S, int p, int iter, Particle& q) {
     const int N = S.N;
-    float P[7], V[7], B[7];
+    const QG float* __restrict__ pos = (const QG float*)S.pos;
+    const QG float* __restrict__ vel = (const QG float*)S.vel;
+    const QG float* __restrict__ pb = (const QG float*)S.pbest;
 #pragma unroll
     for (int d = 0; d < 7; ++d) {
-        P[d] = S.pos[d * N + p];
-        V[d] = S.vel[d * N + p];
-        B[d] = S.pbest[d * N + p];
+        q.P[d] = pos[d * N + p];
+        q.V[d] = vel[d * N + p];
+        q.B[d] = pb[d * N + p];
     }
-    int s = 12 + 12 * iter;  // d_randIdx: 12 at init + 12 per iteration
+    q.fit = ((const QG float*)S.fit)[p];
+    q.pbest_fit = ((const QG float*)S.pbest_fit)[p];
+    // d_randIdx: 12 at init + 12 per iteration; a finished solve (iter = I,
+    // values unused) reads the table's last rows instead of past its end
+    const int s0 = min(12 + 12 * iter, S.S - 12);
+    const QG float* __restrict__ rt = (const QG float*)S.rand;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) q.R[k] = rt[(size_t)p + (size_t)(s0 + k) * (size_t)N];
+}
+
+// UpdateParticlesEquationKernel (main.cu:858-1060) for one particle, gb = gbest
+__device__ void step_particle(const QpsoSolveDev& S, Particle& q, const float* gb) {
+    float* P = q.P;
+    float* V = q.V;
+    const float* B = q.B;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const float r1 = S.c1 * RT(S, p, s); ++s;
-        const float r2 = S.c2 * RT(S, p, s); ++s;
+        const float r1 = S.c1 * q.R[2 * d];
+        const float r2 = S.c2 * q.R[2 * d + 1];
         const float iV = (float)(S.w * (double)V[d]);
         float v = __builtin_fmaf(r1, B[d] - P[d], iV);
         v = __builtin_fmaf(r2, gb[d] - P[d], v);
@@ -978,36 +1026,65 @@ __device__ void update_particle(const QpsoSolveDev& S, int p, int iter, const fl
     V[3] = 0.0f;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        const float r1 = S.c1 * RT(S, p, s); ++s;
-        const float r2 = S.c2 * RT(S, p, s); ++s;
+        const float r1 = S.c1 * q.R[6 + 2 * i];
+        const float r2 = S.c2 * q.R[7 + 2 * i];
         const float iW = (float)(S.w * (double)V[4 + i]);
         const float v = __builtin_fmaf(r1, wc[i], iW);
         V[4 + i] = __builtin_fmaf(r2, ws[i], v);
     }
-    float q[4] = {P[3], P[4], P[5], P[6]};
-    kinematics_from(q, &P[3], &V[4]);
+    float qq[4] = {P[3], P[4], P[5], P[6]};
+    kinematics_from(qq, &P[3], &V[4]);
+}
+
+// personal best (strict <, main.cu:786-811), then the update; stores the particle
+__device__ __forceinline__ void finish_particle(const QpsoSolveDev& S, int p, Particle& q, const float* gb) {
+    const int N = S.N;
+    if (q.fit < q.pbest_fit) {
+        S.pbest_fit[p] = q.fit;
+#pragma unroll
+        for (int d = 0; d < 7; ++d) {
+            q.B[d] = q.P[d];
+            S.pbest[d * N + p] = q.P[d];
+        }
+    }
+    step_particle(S, q, gb);
 #pragma unroll
     for (int d = 0; d < 7; ++d) {
-        S.pos[d * N + p] = P[d];
-        S.vel[d * N + p] = V[d];
+        S.pos[d * N + p] = q.P[d];
+        S.vel[d * N + p] = q.V[d];
     }
 }
 
+// Particle tid's state loads at once with the iteration count and gbest, so
+// the step waits on one round of loads (more when N > BT).  Every thread
+// finishes the argmin from the per-wave winners, and the owner of the winner
+// hands its pose to the others through LDS.
 template <int BT>
 __global__ __launch_bounds__(BT) void k_pso_step(const QpsoSolveDev* __restrict__ solves) {
-    __shared__ float s_v[BT / 64];
-    __shared__ int s_i[BT / 64];
-    __shared__ float s_g[8];
-    __shared__ int s_best;
+    constexpr int kWaves = BT / 64;
+    __shared__ float s_v[kWaves];
+    __shared__ int s_i[kWaves];
+    __shared__ float s_pos[7];
     const QpsoSolveDev& S = solves[blockIdx.x];
     const int tid = threadIdx.x;
-    const int iter = *S.iter;
-    if (iter >= S.I) return;
     const int N = S.N;
+    const int iter = __builtin_amdgcn_readfirstlane(((const QG int32_t*)S.iter)[0]);
+    Particle q0;
+    if (tid < N) load_particle(S, tid, iter, q0);
+    float g[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) g[d] = ((const QG float*)S.gbest)[d];
+    if (iter >= S.I) return;
 
+    // argmin, first index on ties, NaN never chosen (thrust::min_element,
+    // main.cu:1691-1698)
     float bv = 0.0f;
     int bi = INT32_MAX;
-    for (int p = tid; p < N; p += BT) {
+    if (tid < N) {
+        if (q0.fit == q0.fit) { bv = q0.fit; bi = tid; }
+        if (S.fit_trace) S.fit_trace[(size_t)iter * N + tid] = q0.fit;
+    }
+    for (int p = tid + BT; p < N; p += BT) {
         const float v = S.fit[p];
         if (v == v && better(v, p, bv, bi)) { bv = v; bi = p; }
         if (S.fit_trace) S.fit_trace[(size_t)iter * N + p] = v;
@@ -1020,41 +1097,43 @@ __global__ __launch_bounds__(BT) void k_pso_step(const QpsoSolveDev* __restrict_
     }
     if ((tid & 63) == 0) { s_v[tid >> 6] = bv; s_i[tid >> 6] = bi; }
     __syncthreads();
-    if (tid == 0) {
-        for (int k = 1; k < BT / 64; ++k)
-            if (better(s_v[k], s_i[k], bv, bi)) { bv = s_v[k]; bi = s_i[k]; }
-        s_best = bi == INT32_MAX ? 0 : bi;
-    }
-    __syncthreads();
-    const int best = s_best;
+    bv = s_v[0];
+    bi = s_i[0];
+#pragma unroll
+    for (int k = 1; k < kWaves; ++k)
+        if (better(s_v[k], s_i[k], bv, bi)) { bv = s_v[k]; bi = s_i[k]; }
+    const int best = bi == INT32_MAX ? 0 : bi;
 
-    // personal best (strict <, main.cu:786-811)
-    for (int p = tid; p < N; p += BT) {
-        const float v = S.fit[p];
-        if (v < S.pbest_fit[p]) {
-            S.pbest_fit[p] = v;
+    // global best (main.cu:824-851): fit[best] is bv unless every value is
+    // NaN, when no comparison succeeds
+    const bool gnew = bi != INT32_MAX && bv < g[7];
+    if (gnew) {
+        if (tid == best % BT) {
 #pragma unroll
-            for (int d = 0; d < 7; ++d) S.pbest[d * N + p] = S.pos[d * N + p];
+            for (int d = 0; d < 7; ++d) s_pos[d] = best < BT ? q0.P[d] : S.pos[d * N + best];
         }
+        __syncthreads();
+#pragma unroll
+        for (int d = 0; d < 7; ++d) g[d] = s_pos[d];
+        g[7] = bv;
     }
-    // global best (thread 0, main.cu:824-851)
     if (tid == 0) {
-        const float fb = S.fit[best];
-        if (fb < S.gbest[7]) {
-            S.gbest[7] = fb;
+        if (gnew) {
 #pragma unroll
-            for (int d = 0; d < 7; ++d) S.gbest[d] = S.pos[d * N + best];
+            for (int d = 0; d < 8; ++d) S.gbest[d] = g[d];
         }
-#pragma unroll
-        for (int d = 0; d < 8; ++d) s_g[d] = S.gbest[d];
         if (S.gbest_trace) {
 #pragma unroll
-            for (int d = 0; d < 8; ++d) S.gbest_trace[(size_t)iter * 8 + d] = s_g[d];
+            for (int d = 0; d < 8; ++d) S.gbest_trace[(size_t)iter * 8 + d] = g[d];
         }
         if (S.argmin_trace) S.argmin_trace[iter] = best;
     }
-    __syncthreads();
-    for (int p = tid; p < N; p += BT) update_particle(S, p, iter, s_g);
+    if (tid < N) finish_particle(S, tid, q0, g);
+    for (int p = tid + BT; p < N; p += BT) {
+        Particle q;
+        load_particle(S, p, iter, q);
+        finish_particle(S, p, q, g);
+    }
     if (tid == 0) *S.iter = iter + 1;
 }
 
