@@ -156,13 +156,22 @@ typedef struct {
 /* Autotuner: times InitQPSO + `iters` iterations of the listed solves, run
  * together as in a batch, under every workgroup/tile configuration and
  * big-face threshold, and stores the fastest in each solve for its later
- * plain, timed, stage and batched runs.  Results do not depend on the
+ * plain, timed, stage and batched runs; the choice also becomes the
+ * context's default (qpso_set_tuning).  Results do not depend on the
  * choice.  `table` (nullable when cap is 0) receives up to `cap` candidates,
  * `*n_out` (nullable) their number.  The swarm state is left undefined; every
  * run starts with InitQPSO.  No counterpart in the reference, whose launch
  * shapes are compile-time constants (main.cu:1607, main.h:84-99). */
 int qpso_autotune(qpso_ctx* ctx, int n, qpso_solve* const* solves, int iters,
                   qpso_tune_entry* table, int cap, int* n_out);
+
+/* The context's default tuning: the configuration index (0 .. the number
+ * of qpso_autotune candidates per threshold - 1) and big-face threshold
+ * (>= 1) of solves and servers created from now on; a server keeps the
+ * tuning it was created with.  Defaults: configuration 0 (or the
+ * QPSO_RF_CFG environment variable), threshold 96.  No counterpart in the
+ * reference. */
+int qpso_set_tuning(qpso_ctx* ctx, int config, int big_face);
 
 /* Stage entry point (RenderFixedPointKernelAABB + ComputeAABBFinalReduction +
  * ComputeFitnessKernel, main.cu:1633-1681) on caller-given poses [7][n]
@@ -185,7 +194,7 @@ typedef struct {
 } qpso_server_stats;
 
 int qpso_server_create(qpso_ctx* ctx, uint64_t hbm_budget_bytes, qpso_server** out);
-/* Slots: the most particles resident at once (default 4096).  A step runs
+/* Slots: the most particles resident at once (default 8192).  A step runs
  * one iteration of every resident solve, so fewer slots mean shorter steps
  * and lower latency, more slots fuller launches and higher throughput.  A
  * solve with more particles than the slots runs alone.  No counterpart in the

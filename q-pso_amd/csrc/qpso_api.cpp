@@ -794,8 +794,11 @@ int qpso_autotune(qpso_ctx* ctx, int n, qpso_solve* const* solves, int iters, qp
     cleanup();
     if (e != hipSuccess) return set_err(ctx, QPSO_E_HIP, std::string("autotune: ") + hipGetErrorString(e));
     if (n_out) *n_out = k;
-    // apply the choice; cached graphs hold the old configuration
+    // apply the choice to these solves (cached graphs hold the old one) and
+    // make it the default of solves and servers created from now on
     invalidate_plan(ctx);
+    ctx->rf_cfg = best_cfg;
+    ctx->big_face = best_big;
     for (int i = 0; i < n; ++i) {
         qpso_solve* s = solves[i];
         s->rf_cfg = best_cfg;
@@ -803,6 +806,13 @@ int qpso_autotune(qpso_ctx* ctx, int n, qpso_solve* const* solves, int iters, qp
         drop_solve_graph(s);
         HIP_TRY(ctx, hipMemcpy(s->d_desc, &s->h, sizeof(QpsoSolveDev), hipMemcpyHostToDevice));
     }
+    return QPSO_OK;
+}
+
+int qpso_set_tuning(qpso_ctx* ctx, int config, int big_face) {
+    if (!ctx || config < 0 || config >= qpso_launch::rf_num_configs() || big_face < 1) return QPSO_E_INVAL;
+    ctx->rf_cfg = config;
+    ctx->big_face = big_face;
     return QPSO_OK;
 }
 
@@ -870,8 +880,9 @@ struct qpso_server {
     qpso_ctx* ctx = nullptr;
     uint64_t budget = 0;
     uint64_t used = 0;
-    int64_t max_particles = 4096;  // resident particles (slots)
+    int64_t max_particles = 8192;  // resident particles (slots)
     int64_t resident = 0;
+    int rf_cfg = 0, big_face = 96;  // the context's tuning when the server was created
     hipStream_t st = nullptr;
     std::thread th;
     std::mutex mu;
@@ -957,6 +968,8 @@ void server_loop(qpso_server* sv) {
                 sv->done_cv.notify_all();
                 continue;
             }
+            s->rf_cfg = sv->rf_cfg;  // the server's tuning, not the context's current one
+            s->h.big_face = sv->big_face;
             // InitQPSO on the server stream, ahead of the solve's first step
             qpso_launch::init(s->d_desc, 1, sv->st);
             sv->active.push_back({std::move(r), s, 0});
@@ -969,7 +982,7 @@ void server_loop(qpso_server* sv) {
             items.clear();
             win = false;
             for (size_t i = 0; i < sv->active.size(); ++i) {
-                win |= needs_windows(sv->active[i].s, ctx->rf_cfg);
+                win |= needs_windows(sv->active[i].s, sv->rf_cfg);
                 descs.push_back(sv->active[i].s->h);
                 for (int q = 0; q < sv->active[i].s->h.N; ++q) items.push_back(QpsoItem{(int32_t)i, q});
             }
@@ -991,7 +1004,7 @@ void server_loop(qpso_server* sv) {
             changed = false;
         }
         if (launched >= (uint64_t)kInFlight) hipEventSynchronize(ev[launched % kInFlight]);
-        qpso_launch::render_fitness(ctx->rf_cfg, win, d_descs, d_items, (int)items.size(), sv->st);
+        qpso_launch::render_fitness(sv->rf_cfg, win, d_descs, d_items, (int)items.size(), sv->st);
         qpso_launch::pso_step(d_descs, (int)descs.size(), sv->st);
         hipEventRecord(ev[launched % kInFlight], sv->st);
         ++launched;
@@ -1037,6 +1050,8 @@ int qpso_server_create(qpso_ctx* ctx, uint64_t hbm_budget_bytes, qpso_server** o
     auto* sv = new qpso_server();
     sv->ctx = ctx;
     sv->budget = hbm_budget_bytes;
+    sv->rf_cfg = ctx->rf_cfg;
+    sv->big_face = ctx->big_face;
     hipSetDevice(ctx->device);
     if (hipStreamCreateWithFlags(&sv->st, hipStreamNonBlocking) != hipSuccess) {
         delete sv;

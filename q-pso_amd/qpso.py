@@ -23,7 +23,7 @@ EXPORTS = [
     "qpso_params_default", "qpso_create", "qpso_destroy", "qpso_last_error", "qpso_status_str",
     "qpso_mesh_upload", "qpso_mesh_release", "qpso_estimate_pose", "qpso_solve_create",
     "qpso_solve_run", "qpso_solve_get_trace", "qpso_solve_destroy", "qpso_solve_run_timed",
-    "qpso_solve_run_batch", "qpso_autotune",
+    "qpso_solve_run_batch", "qpso_autotune", "qpso_set_tuning",
     "qpso_solve_eval_poses", "qpso_server_create", "qpso_server_set_slots", "qpso_server_submit", "qpso_server_poll",
     "qpso_server_wait", "qpso_server_stats_get", "qpso_server_destroy", "qpso_solve_bytes",
     "qpso_mesh_icosphere", "qpso_mesh_uv_ellipsoid", "qpso_random_quat", "qpso_scene_render",
@@ -116,6 +116,7 @@ def lib():
             "qpso_solve_run_batch": (I32, [P, C.c_int, P, P]),
             "qpso_solve_eval_poses": (I32, [P, C.c_int, P, P, P, P]),
             "qpso_autotune": (I32, [P, C.c_int, P, C.c_int, P, C.c_int, P]),
+            "qpso_set_tuning": (I32, [P, C.c_int, C.c_int]),
             "qpso_server_create": (I32, [P, U64, P]),
             "qpso_server_set_slots": (I32, [P, C.c_int64]),
             "qpso_server_submit": (I32, [P, U32, P, P, P]),
@@ -233,7 +234,8 @@ class Context:
 
     def autotune(self, solves, iters=3):
         """Time every configuration on these solves (run together) and keep
-        the fastest in each; returns the measured table, fastest first."""
+        the fastest in each and as the context's default; returns the
+        measured table, fastest first."""
         arr = (C.c_void_p * len(solves))(*[s.handle for s in solves])
         table = (TuneEntry * 64)()
         n = C.c_int()
@@ -241,6 +243,10 @@ class Context:
         rows = [dict(config=t.config, threads=t.threads, tile_words=t.tile_words, big_face=t.big_face, ms=t.ms)
                 for t in table[:min(n.value, 64)]]
         return sorted(rows, key=lambda r: r["ms"])
+
+    def set_tuning(self, config, big_face):
+        """Default configuration and big-face threshold of solves and servers created from now on."""
+        _check(lib().qpso_set_tuning(self.handle, int(config), int(big_face)), self)
 
 
 class Solve:

@@ -116,3 +116,26 @@ def test_autotune_keeps_results(ctx, frame):
     finally:
         for s in solves:
             s.close()
+
+
+def test_server_uses_context_tuning(ctx, frame):
+    """A server takes the context's tuning when created (here 256 threads,
+    a tile that sends the larger meshes window by window, threshold 160);
+    results stay those of the plain form."""
+    jobs, plain = frame
+    for bad in ((99, 96), (-1, 96), (0, 0)):
+        with pytest.raises(qpso.QpsoError) as e:
+            ctx.set_tuning(*bad)
+        assert e.value.code == qpso.E_INVAL
+    ctx.set_tuning(3, 160)
+    try:
+        sv = qpso.Server(ctx, 8 << 30)
+        ctx.set_tuning(0, 96)  # later changes leave the running server alone
+        try:
+            tickets = [(k, sv.submit(*jobs[k])) for k in range(len(jobs))]
+            for k, t in tickets:
+                assert same(sv.wait(t), plain[k]), f"request for solve {k}"
+        finally:
+            sv.close()
+    finally:
+        ctx.set_tuning(0, 96)
