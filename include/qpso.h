@@ -197,6 +197,8 @@ int qpso_server_create(qpso_ctx* ctx, uint64_t hbm_budget_bytes, qpso_server** o
 /* Slots: the most particles resident at once (default 8192).  A step runs
  * one iteration of every resident solve, so fewer slots mean shorter steps
  * and lower latency, more slots fuller launches and higher throughput.  A
+ * step with at most one particle per CU runs the widest configuration (0),
+ * fuller steps the server's tuning (qpso_set_tuning).  A
  * solve with more particles than the slots runs alone.  No counterpart in the
  * reference (it solves one cluster at a time). */
 int qpso_server_set_slots(qpso_server* sv, int64_t max_resident_particles);

@@ -933,6 +933,7 @@ void server_loop(qpso_server* sv) {
     for (auto& e : ev) hipEventCreateWithFlags(&e, hipEventDisableTiming);
     uint64_t launched = 0;
     bool changed = true, win = false;
+    int step_cfg = sv->rf_cfg;
     for (;;) {
         std::vector<qpso_server::Req> admit;
         {
@@ -982,10 +983,15 @@ void server_loop(qpso_server* sv) {
             items.clear();
             win = false;
             for (size_t i = 0; i < sv->active.size(); ++i) {
-                win |= needs_windows(sv->active[i].s, sv->rf_cfg);
                 descs.push_back(sv->active[i].s->h);
                 for (int q = 0; q < sv->active[i].s->h.N; ++q) items.push_back(QpsoItem{(int32_t)i, q});
             }
+            // A step with no more workgroups than CUs is latency-bound: it
+            // takes the widest configuration (one 1024-thread workgroup per
+            // CU); fuller steps take the server's tuned one.  Results do not
+            // depend on the configuration.
+            step_cfg = items.size() <= (size_t)std::max(ctx->n_cu, 1) ? 0 : sv->rf_cfg;
+            for (size_t i = 0; i < sv->active.size(); ++i) win |= needs_windows(sv->active[i].s, step_cfg);
             if (descs.size() > cap_descs || items.size() > cap_items) {
                 hipStreamSynchronize(sv->st);
                 if (descs.size() > cap_descs) {
@@ -1004,7 +1010,7 @@ void server_loop(qpso_server* sv) {
             changed = false;
         }
         if (launched >= (uint64_t)kInFlight) hipEventSynchronize(ev[launched % kInFlight]);
-        qpso_launch::render_fitness(sv->rf_cfg, win, d_descs, d_items, (int)items.size(), sv->st);
+        qpso_launch::render_fitness(step_cfg, win, d_descs, d_items, (int)items.size(), sv->st);
         qpso_launch::pso_step(d_descs, (int)descs.size(), sv->st);
         hipEventRecord(ev[launched % kInFlight], sv->st);
         ++launched;
