@@ -149,7 +149,7 @@ typedef struct {
     int32_t config;      /* render+fitness configuration index */
     int32_t threads;     /* workgroup size (one workgroup per particle) */
     int32_t tile_words;  /* 32-bit LDS words per workgroup: vertex cache, face queues, depth tile */
-    int32_t big_face;    /* faces over this many clipped pixels are drawn by a whole wave */
+    int32_t big_face;    /* flat faces over this many clipped pixels (others over half) are drawn by a whole wave */
     double ms;           /* InitQPSO + `iters` iterations of the listed solves */
 } qpso_tune_entry;
 

@@ -73,7 +73,7 @@ struct QpsoSolveDev {
     QpsoStale* stale[2];    // [N][stale_cap] each
     int32_t* stale_cnt[2];  // [N] each
     int32_t stale_cap;
-    int32_t big_face;       // faces of more clipped pixels are rasterized by a whole wave
+    int32_t big_face;       // flat faces of more clipped pixels (others: half) are rasterized by a whole wave
     int32_t* flags;         // [1]
     unsigned long long* counters;  // [3]: raster px, visit px, degenerate evals
     // traces (nullable)

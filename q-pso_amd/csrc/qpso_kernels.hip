@@ -234,8 +234,9 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     float* __restrict__ vZ = reinterpret_cast<float*>(lds + 2 * VC);
     int32_t vbase = 0;  // vertex index of cache entry 0
     constexpr int kQCap = CAP / 32;   // small faces needing the per-pixel depth
-    // faces of more than S.big_face pixels, rasterized by a whole wave so one
-    // large face does not hold its wave's other 63 lanes idle: clipped
+    // faces of more than S.big_face pixels (non-flat ones: half as many),
+    // rasterized by a whole wave so one large face does not hold its wave's
+    // other 63 lanes idle: clipped
     // rectangle (x0 | x1 << 16, y0 | y1 << 16), constant depth, face << 1 | flat
     constexpr int kBQCap = CAP / 128;
     int32_t* __restrict__ queue = lds + 3 * VC;
@@ -584,10 +585,11 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         //      ds_min_i32, as one flat pixel loop (a nested row/column loop
         //      would diverge as max rows x max columns); the rest of C into the
         //      mask where its depth is below 60000; and the column and row
-        //      lines.  Faces whose depth over the D part is provably one
-        //      integer (flat_depth) store that constant; the others are queued
+        //      lines.  Faces whose depth over C is provably one integer
+        //      (depth_bounds) store that constant; the others are queued
         //      and rasterized with the per-pixel depth after the pass; D parts
-        //      of more than big_face pixels go to the wave queue.
+        //      of more than big_face pixels (non-flat: big_face / 2) go to the
+        //      wave queue.
         auto draw = [&](int32_t i) {
             Face fc;
             Setup su;
@@ -601,12 +603,12 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             const bool in_d = px0 < px1 && py0 < py1;
             const bool whole = in_d && px0 == x0 && px1 == x1 && py0 == y0 && py1 == y1;
             // bounds over all of C (for the mask and the lines); one depth over
-            // C is one over its D part, else the D part is tested by itself
-            int32_t kmin = 0, kmax = 0, kd = 0;
+            // C is one over its D part (a D part of a face not flat over C
+            // takes the per-pixel path, exact either way)
+            int32_t kmin = 0, kmax = 0;
             const bool bounded = depth_bounds(fc, su, x0, x1, y0, y1, kmin, kmax);
-            bool flat = in_d && bounded && kmin == kmax;
-            if (flat) kd = kmin;
-            else if (in_d && !whole) flat = flat_depth(fc, su, px0, px1, py0, py1, kd);
+            const bool flat = in_d && bounded && kmin == kmax;
+            const int32_t kd = kmin;
             if (MW && !whole) {
                 if (bounded && kmax < QPSO_EMPTY) {
                     mark_rows(x0, x1, y0, y1);
@@ -627,7 +629,9 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             if (!in_d) return;
             const int32_t w = px1 - px0, n = w * (py1 - py0);
             int32_t qb;
-            if (n > big_face && (qb = atomicAdd(&s_nqb, 1)) < kBQCap) {
+            // a non-flat face costs a per-pixel loop in its lane, so it goes to
+            // the wave queue from half the threshold of a flat one
+            if ((flat ? n : 2 * n) > big_face && (qb = atomicAdd(&s_nqb, 1)) < kBQCap) {
                 bq_x[qb] = px0 | (px1 << 16);
                 bq_y[qb] = py0 | (py1 << 16);
                 bq_k[qb] = kd;

@@ -363,12 +363,6 @@ QD bool depth_bounds(const Face& fc, const Setup& s, int32_t x0, int32_t x1, int
     return true;
 }
 
-// The rectangle's written depth is the one integer k.
-QD bool flat_depth(const Face& fc, const Setup& s, int32_t x0, int32_t x1, int32_t y0, int32_t y1, int32_t& k) {
-    int32_t kmax;
-    return depth_bounds(fc, s, x0, x1, y0, y1, k, kmax) && k == kmax;
-}
-
 // Written depth at pixel (x, y) of the face's bbox, from the edge values there.
 QD int32_t depth_at(const Face& fc, const Setup& s, int32_t x, int32_t y) {
     const int32_t CX2 = wsub(wadd(s.CY2, wmul(y - fc.miny, s.FDX23)), wmul(x - fc.minx, s.FDY23));
