@@ -7,13 +7,18 @@ device-resident before timing.  One step = one whole solve (InitQPSO + 100 x
 (render+fitness, argmin/pbest/gbest/update)).  value = solves/s.
 
 Also reported on the same JSON line:
-  roofline      the render+fitness kernel against LDS bandwidth: algorithmic
-                LDS bytes per launch (4 B per bbox-fill atomicMin + 10 B per
-                fitness visit, counted exactly) / measured average launch time
-  roofline_hbm  the same kernel against HBM: unique bytes per launch (mesh,
-                observed ROI, swarm state) / launch time; traffic from rocprof
+  roofline      the render+fitness kernel against HBM: unique bytes per launch
+                (mesh, observed ROI, swarm state) / measured average launch
+                time; traffic = HBM bytes per launch from rocprof
+  roofline_lds  the same kernel against LDS bandwidth: algorithmic LDS bytes
+                per launch (4 B per bbox-fill atomicMin + 10 B per fitness
+                visit, counted exactly) / launch time
+  roofline_valu the same kernel against VALU issue, the resource that binds
+                it: VALU wave instructions per launch (rocprof SQ_INSTS_VALU,
+                committed summary) / launch time
   cpu_baseline  the CPU oracle (C, OpenMP over particles) on a bounded sample
-  batched       16 uneven solves of one frame: batched graph form vs plain
+  batched       16 uneven solves of one frame: batched graph form vs plain,
+                and vs the reference's launch structure (one launch per particle)
   served        Poisson arrivals into the continuous-batching server: p50/p99
 """
 import argparse
@@ -33,6 +38,9 @@ import workloads as W  # noqa: E402
 
 LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz = 78,643 GB/s
 HBM_PEAK_GBS = 8000.0
+# 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 VALU instruction per SIMD every 4
+# clocks (16 lanes wide, packed FP32 off)
+VALU_PEAK_TIPS = 256 * 4 * 2.4e9 / 4 / 1e12
 
 
 def pct(v, q):
@@ -40,14 +48,15 @@ def pct(v, q):
     return v[min(len(v) - 1, int(round(q * (len(v) - 1))))]
 
 
-def traffic_from_profiles():
-    """HBM bytes per render+fitness launch measured by rocprofv3 --pmc (committed summary)."""
+def from_profiles(key):
+    """A per-launch figure of the render+fitness kernel measured by rocprofv3
+    --pmc on the headline workload (committed summary), or None."""
     path = os.path.join(REPO, "profiles", "k_render_fitness_10k_faces_256_particles_pmc.json")
     if not os.path.exists(path):
         return None
     try:
         with open(path) as f:
-            return float(json.load(f)["hbm_bytes_per_launch"])
+            return float(json.load(f)[key])
     except (OSError, ValueError, KeyError):
         return None
 
@@ -85,6 +94,11 @@ def batched_section(ctx, reps=3):
     for _ in range(reps):
         res_p = [s.run() for s in solves]
     t_p = (time.perf_counter() - t) / reps
+    # the reference's launch structure with these kernels: one launch per
+    # particle on 16 streams, a host read every iteration (one pass)
+    t = time.perf_counter()
+    res_r = [s.run_reference_launches() for s in solves]
+    t_r = time.perf_counter() - t
     # batched form: tuned for the whole frame, one graph launch
     tune = ctx.autotune(solves, iters=3)
     ctx.run_batch(solves)  # capture + warm-up
@@ -93,7 +107,7 @@ def batched_section(ctx, reps=3):
         res_b = ctx.run_batch(solves)
     t_b = (time.perf_counter() - t) / reps
     same = all(np.array_equal(np.array(a.pose[:]), np.array(b.pose[:])) and a.fit == b.fit
-               for a, b in zip(res_b, res_p))
+               for a, b in zip(res_b + res_r, res_p + res_p))
     for s in solves:
         s.close()
     for m in meshes:
@@ -101,7 +115,8 @@ def batched_section(ctx, reps=3):
     return {"workload": f"one frame, {n} solves, faces {min(len(o[0]) // 9 for o in scene.objects)}.."
                         f"{max(len(o[0]) // 9 for o in scene.objects)}, particles 64..512, 60 iterations",
             "solves": n, "batched_solves_per_s": n / t_b, "plain_solves_per_s": n / t_p,
-            "speedup": t_p / t_b, "identical_results": bool(same), "autotune": tune[0]}
+            "speedup": t_p / t_b, "reference_launch_solves_per_s": n / t_r,
+            "speedup_vs_reference_launches": t_r / t_b, "identical_results": bool(same), "autotune": tune[0]}
 
 
 def served_section(ctx, lams=(200.0, 1000.0, 4000.0), seconds=1.5, seed=0, slots=8192):
@@ -225,7 +240,7 @@ def main():
         "pixels_per_s": {"raster": r0.raster_px * value, "fitness": r0.visit_px * value},
         "kernel_ms": {"render_fitness_avg": k1_avg_s * 1e3, "pso_step_avg": k2_ms / launches},
         "roofline": {"bound": "hbm", "achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": hbm_gbs / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
+                     "frac": hbm_gbs / HBM_PEAK_GBS, "traffic": from_profiles("hbm_bytes_per_launch"),
                      "algorithmic_bytes_per_launch": hbm_bytes,
                      "kernel": "k_render_fitness"},
         # the binding resource is on-chip: LDS depth-tile traffic (and VALU issue)
@@ -233,6 +248,12 @@ def main():
                          "frac": lds_gbs / LDS_PEAK_GBS, "algorithmic_bytes_per_launch": lds_bytes},
         "autotune": {"chosen": tune[0], "candidates": len(tune), "slowest_ms": tune[-1]["ms"]},
     }
+    valu = from_profiles("valu_instr_per_launch")
+    if valu is not None:
+        tips = valu / k1_avg_s / 1e12
+        out["roofline_valu"] = {"bound": "valu", "achieved": tips, "peak": VALU_PEAK_TIPS,
+                                "unit": "T wave-instr/s", "frac": tips / VALU_PEAK_TIPS,
+                                "instr_per_launch": valu}
     solve.close()
     ctx.release_mesh(mid)
     if not args.no_cpu:

@@ -127,6 +127,15 @@ int qpso_solve_run(qpso_solve* s, qpso_result* out);
 int qpso_solve_get_trace(qpso_solve* s, qpso_trace* tr);
 int qpso_solve_destroy(qpso_solve* s);
 
+/* The same solve with the reference's launch structure, for comparison
+ * only (main.cu:1617-1733): per iteration one render+fitness launch per
+ * particle, particle p on stream p % 16 (NumStreams, main.h:130;
+ * main.cu:1633-1644), then the PSO
+ * step and a host read of the best fitness (the reference copies the
+ * argmin to the host every iteration, main.cu:1691-1698).  The kernels are
+ * this library's, so results equal qpso_solve_run's. */
+int qpso_solve_run_reference_launches(qpso_solve* s, qpso_result* out);
+
 /* qpso_solve_run with HIP events recorded on the launch stream around every
  * kernel (the reference's myTimer brackets, main.cu:1617-1733, per kernel). */
 typedef struct {

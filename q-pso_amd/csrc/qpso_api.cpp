@@ -578,6 +578,55 @@ int qpso_solve_run(qpso_solve* s, qpso_result* out) {
     return read_result(s, out, ctx->stream);
 }
 
+int qpso_solve_run_reference_launches(qpso_solve* s, qpso_result* out) {
+    if (!s) return QPSO_E_INVAL;
+    qpso_ctx* ctx = s->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int N = s->h.N, I = s->h.I;
+    constexpr int kStreams = 16;  // NumStreams (main.h): particle p on stream p % 16
+    std::vector<QpsoItem> items(N);
+    for (int p = 0; p < N; ++p) items[p] = QpsoItem{0, p};
+    QpsoItem* d_items = nullptr;
+    hipStream_t ps[kStreams] = {};
+    hipEvent_t join[kStreams] = {}, fork = nullptr;
+    auto cleanup = [&]() {
+        for (auto& x : ps) if (x) hipStreamDestroy(x);
+        for (auto& x : join) if (x) hipEventDestroy(x);
+        if (fork) hipEventDestroy(fork);
+        if (d_items) hipFree(d_items);
+    };
+    hipError_t e = hipMalloc(&d_items, sizeof(QpsoItem) * N);
+    if (e == hipSuccess) e = hipMemcpy(d_items, items.data(), sizeof(QpsoItem) * N, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
+    for (int k = 0; k < kStreams && e == hipSuccess; ++k) {
+        e = hipStreamCreateWithFlags(&ps[k], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&join[k], hipEventDisableTiming);
+    }
+    const bool win = needs_windows(s, s->rf_cfg);
+    float best = 0.0f;
+    if (e == hipSuccess) e = qpso_launch::init(s->d_desc, 1, st);
+    for (int it = 0; it < I && e == hipSuccess; ++it) {
+        e = hipEventRecord(fork, st);
+        for (int k = 0; k < kStreams && e == hipSuccess; ++k) e = hipStreamWaitEvent(ps[k], fork, 0);
+        for (int p = 0; p < N && e == hipSuccess; ++p)
+            e = qpso_launch::render_fitness(s->rf_cfg, win, s->d_desc, d_items + p, 1, ps[p % kStreams]);
+        for (int k = 0; k < kStreams && e == hipSuccess; ++k) {
+            e = hipEventRecord(join[k], ps[k]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(st, join[k], 0);
+        }
+        if (e == hipSuccess) e = qpso_launch::pso_step(s->d_desc, 1, st);
+        // the reference reads the argmin back to the host every iteration
+        if (e == hipSuccess) e = hipMemcpyAsync(&best, s->arena + s->L.gbest + 7 * sizeof(float), sizeof(float),
+                                                hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+    }
+    cleanup();
+    if (e != hipSuccess) return set_err(ctx, QPSO_E_HIP, std::string("reference-shaped run: ") + hipGetErrorString(e));
+    return read_result(s, out, st);
+}
+
 int qpso_solve_run_timed(qpso_solve* s, qpso_result* out, qpso_timing* tm) {
     if (!s || !tm) return QPSO_E_INVAL;
     qpso_ctx* ctx = s->ctx;

@@ -23,6 +23,7 @@ EXPORTS = [
     "qpso_params_default", "qpso_create", "qpso_destroy", "qpso_last_error", "qpso_status_str",
     "qpso_mesh_upload", "qpso_mesh_release", "qpso_estimate_pose", "qpso_solve_create",
     "qpso_solve_run", "qpso_solve_get_trace", "qpso_solve_destroy", "qpso_solve_run_timed",
+    "qpso_solve_run_reference_launches",
     "qpso_solve_run_batch", "qpso_autotune", "qpso_set_tuning",
     "qpso_solve_eval_poses", "qpso_server_create", "qpso_server_set_slots", "qpso_server_submit", "qpso_server_poll",
     "qpso_server_wait", "qpso_server_stats_get", "qpso_server_destroy", "qpso_solve_bytes",
@@ -113,6 +114,7 @@ def lib():
             "qpso_solve_get_trace": (I32, [P, P]),
             "qpso_solve_destroy": (I32, [P]),
             "qpso_solve_run_timed": (I32, [P, P, P]),
+            "qpso_solve_run_reference_launches": (I32, [P, P]),
             "qpso_solve_run_batch": (I32, [P, C.c_int, P, P]),
             "qpso_solve_eval_poses": (I32, [P, C.c_int, P, P, P, P]),
             "qpso_autotune": (I32, [P, C.c_int, P, C.c_int, P, C.c_int, P]),
@@ -272,6 +274,12 @@ class Solve:
     def run(self):
         r = Result()
         _check(lib().qpso_solve_run(self.handle, C.byref(r)), self.ctx)
+        return r
+
+    def run_reference_launches(self):
+        """The same solve launched the reference's way (one launch per particle, 16 streams, a host read per iteration)."""
+        r = Result()
+        _check(lib().qpso_solve_run_reference_launches(self.handle, C.byref(r)), self.ctx)
         return r
 
     def run_timed(self):

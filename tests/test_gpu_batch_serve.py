@@ -91,6 +91,15 @@ def test_server_result_same_alone_and_in_mix(ctx, frame, slots):
         sv.close()
 
 
+def test_reference_launch_structure_equals_plain(ctx, frame):
+    """One launch per particle over 16 streams (the reference's structure)
+    gives the plain form's results for every solve of the frame."""
+    jobs, plain = frame
+    for k, (mid, cl, p) in enumerate(jobs):
+        with ctx.solve(mid, cl, p) as s:
+            assert same(s.run_reference_launches(), plain[k]), f"solve {k}"
+
+
 def test_server_rejects_over_budget(ctx, frame):
     jobs, _ = frame
     sv = qpso.Server(ctx, 1 << 10)
