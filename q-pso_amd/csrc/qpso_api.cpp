@@ -755,8 +755,8 @@ int qpso_solve_eval_poses(qpso_solve* s, int n, const float* poses, float* fit, 
     if (hipMalloc(&d_aabb, sizeof(int32_t) * 4 * (size_t)n) != hipSuccess) { cleanup(); return QPSO_E_NOMEM; }
     tmp.dbg_aabb = d_aabb;
     if (hipMalloc(&d_tmp, sizeof(QpsoSolveDev)) != hipSuccess) { cleanup(); return QPSO_E_NOMEM; }
-    hipMemcpyAsync(d_tmp, &tmp, sizeof(tmp), hipMemcpyHostToDevice, st);
-    hipError_t e = qpso_launch::render_fitness(s->rf_cfg, needs_windows(s, s->rf_cfg), d_tmp, nullptr, n, st);
+    hipError_t e = hipMemcpyAsync(d_tmp, &tmp, sizeof(tmp), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = qpso_launch::render_fitness(s->rf_cfg, needs_windows(s, s->rf_cfg), d_tmp, nullptr, n, st);
     if (e == hipSuccess && fit) e = hipMemcpyAsync(fit, A + s->L.fit, sizeof(float) * n, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess && aabb) e = hipMemcpyAsync(aabb, d_aabb, sizeof(int32_t) * 4 * n, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess && frames)
@@ -983,6 +983,24 @@ void server_loop(qpso_server* sv) {
     uint64_t launched = 0;
     bool changed = true, win = false;
     int step_cfg = sv->rf_cfg;
+    // a device allocation or launch that fails ends every resident request
+    // with that status instead of leaving its waiter blocked
+    auto fail_active = [&](int code) {
+        hipStreamSynchronize(sv->st);
+        for (auto& a : sv->active) {
+            qpso_solve_destroy(a.s);
+            std::lock_guard<std::mutex> lk(sv->mu);
+            qpso_result res{};
+            res.status = code;
+            sv->results[a.req.ticket] = res;
+            sv->used -= a.req.bytes;
+            sv->resident -= a.req.p.n_particles;
+            ++sv->rejected;
+            sv->done_cv.notify_all();
+        }
+        sv->active.clear();
+        changed = true;
+    };
     for (;;) {
         std::vector<qpso_server::Req> admit;
         {
@@ -1041,26 +1059,47 @@ void server_loop(qpso_server* sv) {
             // depend on the configuration.
             step_cfg = items.size() <= (size_t)std::max(ctx->n_cu, 1) ? 0 : sv->rf_cfg;
             for (size_t i = 0; i < sv->active.size(); ++i) win |= needs_windows(sv->active[i].s, step_cfg);
+            bool ok = true;
             if (descs.size() > cap_descs || items.size() > cap_items) {
                 hipStreamSynchronize(sv->st);
                 if (descs.size() > cap_descs) {
                     if (d_descs) hipFree(d_descs);
-                    cap_descs = descs.size() * 2;
-                    hipMalloc(&d_descs, sizeof(QpsoSolveDev) * cap_descs);
+                    d_descs = nullptr;
+                    cap_descs = 0;
+                    if (hipMalloc(&d_descs, sizeof(QpsoSolveDev) * descs.size() * 2) == hipSuccess)
+                        cap_descs = descs.size() * 2;
+                    else
+                        ok = false;
                 }
-                if (items.size() > cap_items) {
+                if (ok && items.size() > cap_items) {
                     if (d_items) hipFree(d_items);
-                    cap_items = items.size() * 2;
-                    hipMalloc(&d_items, sizeof(QpsoItem) * cap_items);
+                    d_items = nullptr;
+                    cap_items = 0;
+                    if (hipMalloc(&d_items, sizeof(QpsoItem) * items.size() * 2) == hipSuccess)
+                        cap_items = items.size() * 2;
+                    else
+                        ok = false;
+                }
+                if (!ok) {
+                    fail_active(QPSO_E_NOMEM);
+                    continue;
                 }
             }
-            hipMemcpyAsync(d_descs, descs.data(), sizeof(QpsoSolveDev) * descs.size(), hipMemcpyHostToDevice, sv->st);
-            hipMemcpyAsync(d_items, items.data(), sizeof(QpsoItem) * items.size(), hipMemcpyHostToDevice, sv->st);
+            if (hipMemcpyAsync(d_descs, descs.data(), sizeof(QpsoSolveDev) * descs.size(), hipMemcpyHostToDevice,
+                               sv->st) != hipSuccess ||
+                hipMemcpyAsync(d_items, items.data(), sizeof(QpsoItem) * items.size(), hipMemcpyHostToDevice,
+                               sv->st) != hipSuccess) {
+                fail_active(QPSO_E_HIP);
+                continue;
+            }
             changed = false;
         }
         if (launched >= (uint64_t)kInFlight) hipEventSynchronize(ev[launched % kInFlight]);
-        qpso_launch::render_fitness(step_cfg, win, d_descs, d_items, (int)items.size(), sv->st);
-        qpso_launch::pso_step(d_descs, (int)descs.size(), sv->st);
+        if (qpso_launch::render_fitness(step_cfg, win, d_descs, d_items, (int)items.size(), sv->st) != hipSuccess ||
+            qpso_launch::pso_step(d_descs, (int)descs.size(), sv->st) != hipSuccess) {
+            fail_active(QPSO_E_HIP);
+            continue;
+        }
         hipEventRecord(ev[launched % kInFlight], sv->st);
         ++launched;
         ++sv->steps;
