@@ -932,7 +932,8 @@ struct qpso_server {
     int64_t max_particles = 8192;  // resident particles (slots)
     int64_t resident = 0;
     int rf_cfg = 0, big_face = 96;  // the context's tuning when the server was created
-    hipStream_t st = nullptr;
+    static constexpr int kLanes = 2;
+    hipStream_t st[kLanes] = {};    // one stream per lane of resident solves
     std::thread th;
     std::mutex mu;
     std::condition_variable cv, done_cv;
@@ -952,7 +953,6 @@ struct qpso_server {
         qpso_solve* s;
         int iter;  // iterations enqueued so far (every step advances each resident solve by one)
     };
-    std::vector<Active> active;
     std::unordered_map<uint64_t, qpso_result> results;
     std::vector<double> latencies_ms;
     uint64_t next_ticket = 1, submitted = 0, completed = 0, rejected = 0, steps = 0;
@@ -965,29 +965,48 @@ namespace {
 
 // One device step is one PSO iteration of every resident solve: a fused
 // render+fitness launch over all their particles and one PSO launch.  The
-// host knows each solve's iteration count, so steps are enqueued without a
-// round trip; at most kInFlight steps are queued ahead of the device, so a
-// request that arrives joins within about two steps.  The stream is synchronised
-// only when a solve completes and its result is read.
+// resident solves are split into two lanes, each with its own stream, work
+// tables and steps, so the GPU overlaps the tail of one lane's step (its last,
+// shortest workgroups) with the other lane's next step; a new solve joins the
+// lane with fewer particles.  Within a lane the largest meshes' workgroups
+// start first.  The host knows each solve's iteration count, so steps are
+// enqueued without a round trip; at most kInFlight steps per lane are queued
+// ahead of the device, so a request that arrives joins within about two
+// steps.  A lane's stream is synchronised only when one of its solves
+// completes and its result is read.
 void server_loop(qpso_server* sv) {
     constexpr int kInFlight = 2;
+    constexpr int kLanes = qpso_server::kLanes;
     qpso_ctx* ctx = sv->ctx;
     hipSetDevice(ctx->device);
-    QpsoSolveDev* d_descs = nullptr;
-    QpsoItem* d_items = nullptr;
-    size_t cap_descs = 0, cap_items = 0;
-    std::vector<QpsoSolveDev> descs;
-    std::vector<QpsoItem> items;
-    hipEvent_t ev[kInFlight];
-    for (auto& e : ev) hipEventCreateWithFlags(&e, hipEventDisableTiming);
-    uint64_t launched = 0;
-    bool changed = true, win = false;
-    int step_cfg = sv->rf_cfg;
-    // a device allocation or launch that fails ends every resident request
+    struct Lane {
+        hipStream_t st = nullptr;
+        std::vector<qpso_server::Active> active;
+        int64_t particles = 0;
+        QpsoSolveDev* d_descs = nullptr;
+        QpsoItem* d_items = nullptr;
+        size_t cap_descs = 0, cap_items = 0;
+        std::vector<QpsoSolveDev> descs;
+        std::vector<QpsoItem> items;
+        hipEvent_t ev[kInFlight] = {};
+        uint64_t launched = 0;
+        bool changed = true;
+    };
+    Lane lanes[kLanes];
+    for (int k = 0; k < kLanes; ++k) {
+        lanes[k].st = sv->st[k];
+        for (auto& e : lanes[k].ev) hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    }
+    auto any_active = [&]() {
+        for (auto& l : lanes)
+            if (!l.active.empty()) return true;
+        return false;
+    };
+    // a device allocation or launch that fails ends every request of the lane
     // with that status instead of leaving its waiter blocked
-    auto fail_active = [&](int code) {
-        hipStreamSynchronize(sv->st);
-        for (auto& a : sv->active) {
+    auto fail_lane = [&](Lane& l, int code) {
+        hipStreamSynchronize(l.st);
+        for (auto& a : l.active) {
             qpso_solve_destroy(a.s);
             std::lock_guard<std::mutex> lk(sv->mu);
             qpso_result res{};
@@ -998,15 +1017,56 @@ void server_loop(qpso_server* sv) {
             ++sv->rejected;
             sv->done_cv.notify_all();
         }
-        sv->active.clear();
-        changed = true;
+        l.active.clear();
+        l.particles = 0;
+        l.changed = true;
+    };
+    // the work tables of a lane: its solves' descriptors and one item per
+    // particle, the largest meshes' particles first
+    auto rebuild = [&](Lane& l) {
+        l.descs.clear();
+        l.items.clear();
+        std::vector<int> order(l.active.size());
+        for (size_t i = 0; i < l.active.size(); ++i) {
+            l.descs.push_back(l.active[i].s->h);
+            order[i] = (int)i;
+        }
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return l.descs[a].F_eff > l.descs[b].F_eff; });
+        for (int i : order)
+            for (int q = 0; q < l.descs[i].N; ++q) l.items.push_back(QpsoItem{(int32_t)i, q});
+        if (l.descs.size() > l.cap_descs || l.items.size() > l.cap_items) {
+            hipStreamSynchronize(l.st);  // the old tables may still be read by queued steps
+            if (l.descs.size() > l.cap_descs) {
+                if (l.d_descs) hipFree(l.d_descs);
+                l.d_descs = nullptr;
+                l.cap_descs = 0;
+                if (hipMalloc(&l.d_descs, sizeof(QpsoSolveDev) * l.descs.size() * 2) != hipSuccess) return QPSO_E_NOMEM;
+                l.cap_descs = l.descs.size() * 2;
+            }
+            if (l.items.size() > l.cap_items) {
+                if (l.d_items) hipFree(l.d_items);
+                l.d_items = nullptr;
+                l.cap_items = 0;
+                if (hipMalloc(&l.d_items, sizeof(QpsoItem) * l.items.size() * 2) != hipSuccess) return QPSO_E_NOMEM;
+                l.cap_items = l.items.size() * 2;
+            }
+        }
+        // stream-ordered behind the steps that read the old tables
+        if (hipMemcpyAsync(l.d_descs, l.descs.data(), sizeof(QpsoSolveDev) * l.descs.size(), hipMemcpyHostToDevice,
+                           l.st) != hipSuccess ||
+            hipMemcpyAsync(l.d_items, l.items.data(), sizeof(QpsoItem) * l.items.size(), hipMemcpyHostToDevice,
+                           l.st) != hipSuccess)
+            return QPSO_E_HIP;
+        l.changed = false;
+        return QPSO_OK;
     };
     for (;;) {
         std::vector<qpso_server::Req> admit;
         {
             std::unique_lock<std::mutex> lk(sv->mu);
-            sv->cv.wait(lk, [&] { return sv->stop || !sv->pending.empty() || !sv->active.empty(); });
-            if (sv->stop && sv->active.empty()) break;
+            sv->cv.wait(lk, [&] { return sv->stop || !sv->pending.empty() || any_active(); });
+            if (sv->stop && !any_active()) break;
             // admit in arrival order while the budget and the slots allow (a
             // solve larger than the slots runs when nothing else is resident)
             while (!sv->pending.empty()) {
@@ -1038,101 +1098,83 @@ void server_loop(qpso_server* sv) {
             }
             s->rf_cfg = sv->rf_cfg;  // the server's tuning, not the context's current one
             s->h.big_face = sv->big_face;
-            // InitQPSO on the server stream, ahead of the solve's first step
-            qpso_launch::init(s->d_desc, 1, sv->st);
-            sv->active.push_back({std::move(r), s, 0});
-            changed = true;
+            Lane& l = lanes[0].particles <= lanes[1].particles ? lanes[0] : lanes[1];
+            // InitQPSO on the lane's stream, ahead of the solve's first step
+            qpso_launch::init(s->d_desc, 1, l.st);
+            l.particles += s->h.N;
+            l.active.push_back({std::move(r), s, 0});
+            l.changed = true;
         }
-        if (sv->active.empty()) continue;
+        if (!any_active()) continue;
 
-        if (changed) {  // the copies are stream-ordered behind the steps that read the old arrays
-            descs.clear();
-            items.clear();
-            win = false;
-            for (size_t i = 0; i < sv->active.size(); ++i) {
-                descs.push_back(sv->active[i].s->h);
-                for (int q = 0; q < sv->active[i].s->h.N; ++q) items.push_back(QpsoItem{(int32_t)i, q});
-            }
-            // A step with no more workgroups than CUs is latency-bound: it
-            // takes the widest configuration (one 1024-thread workgroup per
-            // CU); fuller steps take the server's tuned one.  Results do not
-            // depend on the configuration.
-            step_cfg = items.size() <= (size_t)std::max(ctx->n_cu, 1) ? 0 : sv->rf_cfg;
-            for (size_t i = 0; i < sv->active.size(); ++i) win |= needs_windows(sv->active[i].s, step_cfg);
-            bool ok = true;
-            if (descs.size() > cap_descs || items.size() > cap_items) {
-                hipStreamSynchronize(sv->st);
-                if (descs.size() > cap_descs) {
-                    if (d_descs) hipFree(d_descs);
-                    d_descs = nullptr;
-                    cap_descs = 0;
-                    if (hipMalloc(&d_descs, sizeof(QpsoSolveDev) * descs.size() * 2) == hipSuccess)
-                        cap_descs = descs.size() * 2;
-                    else
-                        ok = false;
-                }
-                if (ok && items.size() > cap_items) {
-                    if (d_items) hipFree(d_items);
-                    d_items = nullptr;
-                    cap_items = 0;
-                    if (hipMalloc(&d_items, sizeof(QpsoItem) * items.size() * 2) == hipSuccess)
-                        cap_items = items.size() * 2;
-                    else
-                        ok = false;
-                }
-                if (!ok) {
-                    fail_active(QPSO_E_NOMEM);
+        // A step with no more workgroups than CUs is latency-bound: it takes
+        // the widest configuration (one 1024-thread workgroup per CU); fuller
+        // steps take the server's tuned one.  Results do not depend on it.
+        int64_t total = 0;
+        size_t n_active = 0;
+        for (auto& l : lanes) {
+            total += l.particles;
+            n_active += l.active.size();
+        }
+        const int step_cfg = total <= (int64_t)std::max(ctx->n_cu, 1) ? 0 : sv->rf_cfg;
+        ++sv->steps;
+        sv->active_sum += (double)n_active;
+        for (auto& l : lanes) {
+            if (l.active.empty()) continue;
+            if (l.changed) {
+                if (const int rc = rebuild(l)) {
+                    fail_lane(l, rc);
                     continue;
                 }
             }
-            if (hipMemcpyAsync(d_descs, descs.data(), sizeof(QpsoSolveDev) * descs.size(), hipMemcpyHostToDevice,
-                               sv->st) != hipSuccess ||
-                hipMemcpyAsync(d_items, items.data(), sizeof(QpsoItem) * items.size(), hipMemcpyHostToDevice,
-                               sv->st) != hipSuccess) {
-                fail_active(QPSO_E_HIP);
+            bool win = false;
+            for (auto& a : l.active) win |= needs_windows(a.s, step_cfg);
+            if (l.launched >= (uint64_t)kInFlight) hipEventSynchronize(l.ev[l.launched % kInFlight]);
+            if (qpso_launch::render_fitness(step_cfg, win, l.d_descs, l.d_items, (int)l.items.size(), l.st) !=
+                    hipSuccess ||
+                qpso_launch::pso_step(l.d_descs, (int)l.descs.size(), l.st) != hipSuccess) {
+                fail_lane(l, QPSO_E_HIP);
                 continue;
             }
-            changed = false;
+            hipEventRecord(l.ev[l.launched % kInFlight], l.st);
+            ++l.launched;
         }
-        if (launched >= (uint64_t)kInFlight) hipEventSynchronize(ev[launched % kInFlight]);
-        if (qpso_launch::render_fitness(step_cfg, win, d_descs, d_items, (int)items.size(), sv->st) != hipSuccess ||
-            qpso_launch::pso_step(d_descs, (int)descs.size(), sv->st) != hipSuccess) {
-            fail_active(QPSO_E_HIP);
-            continue;
+        for (auto& l : lanes) {
+            bool any_done = false;
+            for (auto& a : l.active) any_done |= ++a.iter >= a.s->h.I;
+            if (!any_done) continue;
+            hipStreamSynchronize(l.st);
+            std::vector<qpso_server::Active> keep;
+            for (auto& a : l.active) {
+                if (a.iter < a.s->h.I) {
+                    keep.push_back(std::move(a));
+                    continue;
+                }
+                qpso_result res{};
+                const int rc = read_result(a.s, &res, l.st);
+                if (rc && res.status == QPSO_OK) res.status = rc;
+                const auto now = std::chrono::steady_clock::now();
+                l.particles -= a.s->h.N;
+                qpso_solve_destroy(a.s);
+                std::lock_guard<std::mutex> lk(sv->mu);
+                sv->results[a.req.ticket] = res;
+                sv->latencies_ms.push_back(std::chrono::duration<double, std::milli>(now - a.req.t_submit).count());
+                sv->used -= a.req.bytes;
+                sv->resident -= a.req.p.n_particles;
+                ++sv->completed;
+                sv->t_last = now;
+                sv->done_cv.notify_all();
+            }
+            l.active.swap(keep);
+            l.changed = true;
         }
-        hipEventRecord(ev[launched % kInFlight], sv->st);
-        ++launched;
-        ++sv->steps;
-        sv->active_sum += (double)sv->active.size();
-
-        bool any_done = false;
-        for (auto& a : sv->active) any_done |= ++a.iter >= a.s->h.I;
-        if (!any_done) continue;
-        hipStreamSynchronize(sv->st);
-        std::vector<qpso_server::Active> keep;
-        for (auto& a : sv->active) {
-            if (a.iter < a.s->h.I) { keep.push_back(std::move(a)); continue; }
-            qpso_result res{};
-            const int rc = read_result(a.s, &res, sv->st);
-            if (rc && res.status == QPSO_OK) res.status = rc;
-            const auto now = std::chrono::steady_clock::now();
-            qpso_solve_destroy(a.s);
-            std::lock_guard<std::mutex> lk(sv->mu);
-            sv->results[a.req.ticket] = res;
-            sv->latencies_ms.push_back(std::chrono::duration<double, std::milli>(now - a.req.t_submit).count());
-            sv->used -= a.req.bytes;
-            sv->resident -= a.req.p.n_particles;
-            ++sv->completed;
-            sv->t_last = now;
-            sv->done_cv.notify_all();
-        }
-        sv->active.swap(keep);
-        changed = true;
     }
-    hipStreamSynchronize(sv->st);
-    for (auto& e : ev) hipEventDestroy(e);
-    if (d_descs) hipFree(d_descs);
-    if (d_items) hipFree(d_items);
+    for (auto& l : lanes) {
+        hipStreamSynchronize(l.st);
+        for (auto& e : l.ev) hipEventDestroy(e);
+        if (l.d_descs) hipFree(l.d_descs);
+        if (l.d_items) hipFree(l.d_items);
+    }
 }
 
 }  // namespace
@@ -1147,9 +1189,13 @@ int qpso_server_create(qpso_ctx* ctx, uint64_t hbm_budget_bytes, qpso_server** o
     sv->rf_cfg = ctx->rf_cfg;
     sv->big_face = ctx->big_face;
     hipSetDevice(ctx->device);
-    if (hipStreamCreateWithFlags(&sv->st, hipStreamNonBlocking) != hipSuccess) {
-        delete sv;
-        return QPSO_E_HIP;
+    for (auto& x : sv->st) {
+        if (hipStreamCreateWithFlags(&x, hipStreamNonBlocking) != hipSuccess) {
+            for (auto& y : sv->st)
+                if (y) hipStreamDestroy(y);
+            delete sv;
+            return QPSO_E_HIP;
+        }
     }
     sv->th = std::thread(server_loop, sv);
     *out = sv;
@@ -1242,7 +1288,7 @@ int qpso_server_destroy(qpso_server* sv) {
     }
     sv->cv.notify_all();
     if (sv->th.joinable()) sv->th.join();
-    hipStreamDestroy(sv->st);
+    for (auto& x : sv->st) hipStreamDestroy(x);
     delete sv;
     return QPSO_OK;
 }
