@@ -779,12 +779,11 @@ int qpso_autotune(qpso_ctx* ctx, int n, qpso_solve* const* solves, int iters, qp
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     // the solves run together as in a batch: descriptors with the candidate
-    // threshold, particles of the heaviest solves first
+    // threshold, particles of the largest meshes first
     std::vector<QpsoSolveDev> descs(n);
     std::vector<int> order(n);
     for (int i = 0; i < n; ++i) { descs[i] = solves[i]->h; order[i] = i; }
-    auto cost = [&](int i) { return (double)descs[i].F_eff * descs[i].N; };
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return descs[a].F_eff > descs[b].F_eff; });
     std::vector<QpsoItem> items;
     for (int i : order)
         for (int q = 0; q < descs[i].N; ++q) items.push_back(QpsoItem{i, q});
@@ -885,9 +884,9 @@ int qpso_solve_run_batch(qpso_ctx* ctx, int n, qpso_solve* const* solves, qpso_r
         std::vector<QpsoSolveDev> descs(n);
         std::vector<int> order(n);
         for (int i = 0; i < n; ++i) { descs[i] = solves[i]->h; order[i] = i; P.max_iters = std::max(P.max_iters, solves[i]->h.I); }
-        // heaviest solves first so the longest workgroups start early
-        auto cost = [&](int i) { return (double)descs[i].F_eff * descs[i].N; };
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+        // the largest meshes' workgroups first (a workgroup's time grows with
+        // its mesh), so the launch ends on short ones
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return descs[a].F_eff > descs[b].F_eff; });
         std::vector<QpsoItem> items;
         for (int i : order)
             for (int q = 0; q < descs[i].N; ++q) items.push_back(QpsoItem{i, q});
