@@ -297,8 +297,10 @@ int read_result(qpso_solve* s, qpso_result* out, hipStream_t st) {
         out->raster_px = c[0];
         out->visit_px = c[1];
         out->n_degenerate = (uint32_t)c[2];
-        out->status = (flags & QPSO_FLAG_STALE_OVERFLOW) ? QPSO_E_STALE : QPSO_OK;
+        out->status = (flags & QPSO_FLAG_LAUNCH) ? QPSO_E_HIP
+                      : (flags & QPSO_FLAG_STALE_OVERFLOW) ? QPSO_E_STALE : QPSO_OK;
     }
+    if (flags & QPSO_FLAG_LAUNCH) return set_err(ctx, QPSO_E_HIP, "render kernel launched without mesh windows");
     return (flags & QPSO_FLAG_STALE_OVERFLOW) ? set_err(ctx, QPSO_E_STALE, "stale-pixel list overflow")
                                                : QPSO_OK;
 }

@@ -202,6 +202,13 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         }
     }
     if (iter >= S.I) return;  // this solve has finished (batched runs)
+    if (!WIN && Ve > kWin) {  // launched only for meshes that fit the vertex cache
+        if (tid == 0) {
+            S.fit[p] = __builtin_nanf("");
+            atomicOr(S.flags, QPSO_FLAG_LAUNCH);
+        }
+        return;
+    }
 
     const Cam cam{S.fx, S.fy, S.cx, S.cy};
     const float* __restrict__ mesh = S.mesh;
@@ -226,7 +233,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     // faces whose corners lie in a window of at most kWin vertex indices;
     // each band then projects the windows in turn.  Without usable runs every
     // face's corners are projected where needed.
-    const bool cached = Ve <= kWin;
+    const bool cached = !WIN || Ve <= kWin;  // compile-time true without windows
     const bool runs = WIN && !cached && S.n_runs > 0 && S.n_runs * 256 <= Fe;  // runs of 256 faces on average
     const int VC = cached ? Ve : (runs ? kWin : 0);  // cache capacity in vertices
     int32_t* __restrict__ vX = lds;
@@ -277,7 +284,8 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
                 const int v = tid + j * BT;
                 if (v < Ve) take(v, ux[j], uy[j], uz[j]);
             }
-            for (int v = tid + kVPT * BT; v < Ve; v += BT) take(v, wx[v], wy[v], wz[v]);  // beyond the cache
+            if (WIN)
+                for (int v = tid + kVPT * BT; v < Ve; v += BT) take(v, wx[v], wy[v], wz[v]);  // beyond the cache
             if (xl <= xh) {
                 bx0 = min(bx0, wadd(xl, 15) >> 4); bx1 = max(bx1, wadd(xh, 15) >> 4);
                 by0 = min(by0, wadd(yl, 15) >> 4); by1 = max(by1, wadd(yh, 15) >> 4);
