@@ -392,7 +392,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     };
     // the face's 28.4 vertices: from the LDS vertex cache, or projected again
     auto load_face = [&](const QpsoFaceRef& e, Face& fc) {
-        if (VC > 0) {
+        if (!WIN || VC > 0) {  // always cached without windows
             const int32_t vb = WIN ? vbase : 0;
             const int32_t a = e.v0 - vb, b = e.v1 - vb, c = e.v2 - vb;
             fc.X[0] = vX[a]; fc.Y[0] = vY[a]; fc.Z[0] = vZ[a];
