@@ -15,7 +15,8 @@
 // flags[] bits
 #define QPSO_FLAG_STALE_OVERFLOW 1
 // a workgroup was launched with an instantiation that cannot hold its mesh
-// (the host never does this; the workgroup writes NaN and stops)
+// or dump its frame (the host never does this; the workgroup writes NaN and
+// stops)
 #define QPSO_FLAG_LAUNCH 2
 
 // carried-over pixels kept per particle and iteration; more set

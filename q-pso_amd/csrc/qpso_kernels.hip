@@ -202,7 +202,8 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         }
     }
     if (iter >= S.I) return;  // this solve has finished (batched runs)
-    if (!WIN && Ve > kWin) {  // launched only for meshes that fit the vertex cache
+    // launched only for meshes that fit the vertex cache, and without frames
+    if (!WIN && (Ve > kWin || S.dbg_frames)) {
         if (tid == 0) {
             S.fit[p] = __builtin_nanf("");
             atomicOr(S.flags, QPSO_FLAG_LAUNCH);
@@ -372,7 +373,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     // next iteration: those two lines keep exact depth in their own arrays.
     // A stage call that dumps the rendered frames keeps the depth of the whole
     // tile (D = tile, no mask).
-    const bool full = S.dbg_frames != nullptr;
+    const bool full = WIN && S.dbg_frames != nullptr;  // frames: the window instantiation only
     const int32_t dx0 = full ? tx0 : max(tx0, rx0), dx1 = full ? tx1 : min(tx1, rx1);
     const int32_t dy0 = full ? ty0 : max(ty0, ry0), dy1 = full ? ty1 : min(ty1, ry1);
     const int32_t DW = dx0 < dx1 && dy0 < dy1 ? dx1 - dx0 : 0;
