@@ -107,7 +107,10 @@ const char* qpso_last_error(qpso_ctx* ctx);
 const char* qpso_status_str(int status);
 
 /* Mesh upload (loadOBJ3 + cudaMalloc/cudaMemcpy of d_obj_model,
- * main.cu:1366-1373).  The mesh stays resident until released. */
+ * main.cu:1366-1373).  The mesh stays resident until released.  A release
+ * while solves created on the mesh exist takes effect when the last of them
+ * is destroyed; from the release on, new solves on it fail with
+ * QPSO_E_NOTFOUND. */
 int qpso_mesh_upload(qpso_ctx* ctx, const float* soa, uint32_t n_faces, uint32_t* mesh_id);
 int qpso_mesh_release(qpso_ctx* ctx, uint32_t mesh_id);
 

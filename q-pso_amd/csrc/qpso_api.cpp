@@ -75,7 +75,17 @@ struct Mesh {
     std::vector<uint32_t> v_prefix;  // unique vertices referenced by faces [0, f+1)
     int32_t* runs = nullptr;         // (first face, first vertex, end vertex) per run
     int n_runs = 0;
+    int refs = 0;           // solves created on the mesh and not yet destroyed
+    bool released = false;  // qpso_mesh_release called; freed when refs reaches 0
 };
+
+void free_mesh(Mesh& m) {
+    hipFree(m.d);
+    hipFree(m.verts);
+    hipFree(m.faces);
+    hipFree(m.runs);
+    m = Mesh{};
+}
 
 // Deduplicate the corners of the non-indexed SoA mesh (bitwise-equal
 // coordinates become one vertex, numbered by first use in face order, so the
@@ -409,12 +419,7 @@ int qpso_destroy(qpso_ctx* ctx) {
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
     invalidate_plan(ctx);
-    for (auto& kv : ctx->meshes) {
-        hipFree(kv.second.d);
-        hipFree(kv.second.verts);
-        hipFree(kv.second.faces);
-        hipFree(kv.second.runs);
-    }
+    for (auto& kv : ctx->meshes) free_mesh(kv.second);
     for (auto& kv : ctx->rand_cache) hipFree(kv.second);
     hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -435,14 +440,21 @@ int qpso_mesh_upload(qpso_ctx* ctx, const float* soa, uint32_t n_faces, uint32_t
     build_mesh_tables(soa, n_faces, verts, order, m.v_prefix, runs);
     m.n_runs = (int)(runs.size() / 3);
     m.V = (uint32_t)(verts.size() / 3);
-    HIP_TRY(ctx, hipMalloc(&m.d, sizeof(float) * 9 * (size_t)n_faces));
-    HIP_TRY(ctx, hipMalloc(&m.verts, sizeof(float) * verts.size()));
-    HIP_TRY(ctx, hipMalloc(&m.faces, sizeof(QpsoFaceRef) * order.size()));
-    HIP_TRY(ctx, hipMemcpy(m.d, soa, sizeof(float) * 9 * (size_t)n_faces, hipMemcpyHostToDevice));
-    HIP_TRY(ctx, hipMemcpy(m.verts, verts.data(), sizeof(float) * verts.size(), hipMemcpyHostToDevice));
-    HIP_TRY(ctx, hipMemcpy(m.faces, order.data(), sizeof(QpsoFaceRef) * order.size(), hipMemcpyHostToDevice));
-    HIP_TRY(ctx, hipMalloc(&m.runs, sizeof(int32_t) * runs.size()));
-    HIP_TRY(ctx, hipMemcpy(m.runs, runs.data(), sizeof(int32_t) * runs.size(), hipMemcpyHostToDevice));
+    hipError_t e = hipMalloc(&m.d, sizeof(float) * 9 * (size_t)n_faces);
+    if (e == hipSuccess) e = hipMalloc(&m.verts, sizeof(float) * verts.size());
+    if (e == hipSuccess) e = hipMalloc(&m.faces, sizeof(QpsoFaceRef) * order.size());
+    if (e == hipSuccess) e = hipMalloc(&m.runs, sizeof(int32_t) * std::max<size_t>(runs.size(), 1));
+    if (e == hipSuccess) e = hipMemcpy(m.d, soa, sizeof(float) * 9 * (size_t)n_faces, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(m.verts, verts.data(), sizeof(float) * verts.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(m.faces, order.data(), sizeof(QpsoFaceRef) * order.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !runs.empty())
+        e = hipMemcpy(m.runs, runs.data(), sizeof(int32_t) * runs.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        free_mesh(m);
+        return set_err(ctx, e == hipErrorOutOfMemory ? QPSO_E_NOMEM : QPSO_E_HIP,
+                       std::string("mesh upload: ") + hipGetErrorString(e));
+    }
     *mesh_id = ctx->next_mesh++;
     ctx->meshes[*mesh_id] = std::move(m);
     return QPSO_OK;
@@ -452,12 +464,12 @@ int qpso_mesh_release(qpso_ctx* ctx, uint32_t mesh_id) {
     if (!ctx) return QPSO_E_INVAL;
     std::lock_guard<std::mutex> g(ctx->mu);
     auto it = ctx->meshes.find(mesh_id);
-    if (it == ctx->meshes.end()) return QPSO_E_NOTFOUND;
+    if (it == ctx->meshes.end() || it->second.released) return QPSO_E_NOTFOUND;
+    // solves still created on the mesh keep it until the last is destroyed
+    it->second.released = true;
+    if (it->second.refs > 0) return QPSO_OK;
     hipStreamSynchronize(ctx->stream);
-    hipFree(it->second.d);
-    hipFree(it->second.verts);
-    hipFree(it->second.faces);
-    hipFree(it->second.runs);
+    free_mesh(it->second);
     ctx->meshes.erase(it);
     return QPSO_OK;
 }
@@ -476,7 +488,7 @@ int qpso_solve_create(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl, c
     std::lock_guard<std::mutex> g(ctx->mu);
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     auto mit = ctx->meshes.find(mesh_id);
-    if (mit == ctx->meshes.end()) return set_err(ctx, QPSO_E_NOTFOUND, "unknown mesh id");
+    if (mit == ctx->meshes.end() || mit->second.released) return set_err(ctx, QPSO_E_NOTFOUND, "unknown mesh id");
     const int N = p->n_particles, I = p->n_iters, S = rand_stride(I);
     const int rw = cl->rect[2] - cl->rect[0], rh = cl->rect[3] - cl->rect[1];
 
@@ -552,6 +564,7 @@ int qpso_solve_create(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl, c
         hipStreamSynchronize(st) != hipSuccess)
         rc = set_err(ctx, QPSO_E_HIP, "solve upload failed");
     if (rc) { hipFree(s->arena); return rc; }
+    ++mit->second.refs;
     *out = s.release();
     return QPSO_OK;
 }
@@ -567,6 +580,11 @@ int qpso_solve_destroy(qpso_solve* s) {
     drop_solve_graph(s);
     for (auto& e : s->timed_ev) hipEventDestroy(e);
     hipFree(s->arena);
+    auto mit = ctx->meshes.find(s->mesh_id);
+    if (mit != ctx->meshes.end() && --mit->second.refs == 0 && mit->second.released) {
+        free_mesh(mit->second);
+        ctx->meshes.erase(mit);
+    }
     delete s;
     return QPSO_OK;
 }

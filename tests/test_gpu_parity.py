@@ -180,3 +180,26 @@ def test_tile_configs_agree(scene1, cfg):
             del os.environ["QPSO_RF_CFG"]
         else:
             os.environ["QPSO_RF_CFG"] = old
+
+
+def test_mesh_release_waits_for_its_solves(ctx, scene0):
+    """Releasing a mesh that a solve still uses defers the free: the solve
+    runs as before, new solves on the mesh are refused, and the mesh goes
+    with the last solve."""
+    scene, params = scene0
+    mid = ctx.upload_mesh(scene.mesh_m())
+    s = ctx.solve(mid, scene.cluster(), params)
+    try:
+        before = s.run()
+        ctx.release_mesh(mid)
+        after = s.run()
+        assert np.float32(before.fit).view(np.uint32) == np.float32(after.fit).view(np.uint32)
+        assert np.array_equal(np.array(before.pose[:], np.float32), np.array(after.pose[:], np.float32))
+        with pytest.raises(qpso.QpsoError) as e:
+            ctx.solve(mid, scene.cluster(), params)
+        assert e.value.code == qpso.E_NOTFOUND
+    finally:
+        s.close()
+    with pytest.raises(qpso.QpsoError) as e:
+        ctx.release_mesh(mid)
+    assert e.value.code == qpso.E_NOTFOUND
