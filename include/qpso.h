@@ -55,8 +55,8 @@ typedef struct {
 /* Per-solve PSO parameters.  The reference fixes these at compile time
  * (main.h:83-105, QPSOParams_t main.h:182-204, filled at main.cu:1303-1331). */
 typedef struct {
-    int32_t n_particles;   /* Nparticle_ (1024) */
-    int32_t n_iters;       /* max_steps_ (20) */
+    int32_t n_particles;   /* Nparticle_ (1024); at most 65535 */
+    int32_t n_iters;       /* max_steps_ (20); at most 2^20 */
     double w;              /* psoW_ (0.3): inertia, applied in binary64 */
     float c1, c2;          /* c1_, c2_ (1.0f) */
     float t_bound;         /* x/y translation half-range around the centroid, m (0.3) */

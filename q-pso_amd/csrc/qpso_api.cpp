@@ -34,6 +34,8 @@ constexpr int kRows = QPSO_ROWS;
 constexpr int kCols = QPSO_COLS;
 constexpr int kFrame = kRows * kCols;
 constexpr int kStaleCap = QPSO_STALE_CAP;
+constexpr int kMaxIters = 1 << 20;          // iterations per solve
+constexpr size_t kMaxRand = size_t(1) << 30;  // random-table entries (4 GiB)
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
@@ -260,7 +262,8 @@ int set_err(qpso_ctx* ctx, int code, const std::string& msg) {
 
 int validate(const qpso_cluster* cl, const qpso_params* p) {
     if (!cl || !p || !cl->depth) return QPSO_E_INVAL;
-    if (p->n_particles <= 0 || p->n_particles > 65535 || p->n_iters <= 0) return QPSO_E_INVAL;
+    if (p->n_particles <= 0 || p->n_particles > 65535 || p->n_iters <= 0 || p->n_iters > kMaxIters)
+        return QPSO_E_INVAL;
     const int32_t* r = cl->rect;
     if (r[0] < 0 || r[1] < 0 || r[2] > kCols || r[3] > kRows || r[2] <= r[0] || r[3] <= r[1])
         return QPSO_E_INVAL;
@@ -273,12 +276,16 @@ int get_rand(qpso_ctx* ctx, uint32_t seed, int N, int S, float** out) {
     auto key = std::make_tuple(seed, N, S);
     auto it = ctx->rand_cache.find(key);
     if (it != ctx->rand_cache.end()) { *out = it->second; return QPSO_OK; }
+    if ((size_t)S * N > kMaxRand) return set_err(ctx, QPSO_E_NOMEM, "random table too large");
     std::vector<float> h((size_t)S * N);
     Mt19937 mt(seed);
     for (auto& v : h) v = (float)((double)mt.next() / 4294967296.0);
     float* d = nullptr;
     HIP_TRY(ctx, hipMalloc(&d, h.size() * sizeof(float)));
-    HIP_TRY(ctx, hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    if (hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(d);
+        return set_err(ctx, QPSO_E_HIP, "random table upload failed");
+    }
     ctx->rand_cache[key] = d;
     *out = d;
     return QPSO_OK;

@@ -203,3 +203,35 @@ def test_mesh_release_waits_for_its_solves(ctx, scene0):
     with pytest.raises(qpso.QpsoError) as e:
         ctx.release_mesh(mid)
     assert e.value.code == qpso.E_NOTFOUND
+
+
+@pytest.mark.parametrize("case", ["rect_outside", "rect_empty", "no_points", "no_particles",
+                                  "too_many_particles", "no_iters", "too_many_iters", "bad_mesh"])
+def test_invalid_inputs_are_refused(ctx, scene0, case):
+    """Bad clusters, swarm sizes and mesh ids return a status, with nothing launched."""
+    scene, params = scene0
+    c = scene.clusters[0]
+    rect, npts = list(c["rect"]), c["num_points"]
+    p = qpso.make_params(params.n_particles, params.n_iters, cam=W.CAM)
+    mid = ctx.upload_mesh(scene.mesh_m())
+    try:
+        if case == "rect_outside":
+            rect = [rect[0], rect[1], 700, rect[3]]
+        elif case == "rect_empty":
+            rect = [rect[0], rect[1], rect[0], rect[3]]
+        elif case == "no_points":
+            npts = 0
+        elif case == "no_particles":
+            p.n_particles = 0
+        elif case == "too_many_particles":
+            p.n_particles = 65536
+        elif case == "no_iters":
+            p.n_iters = 0
+        elif case == "too_many_iters":
+            p.n_iters = (1 << 20) + 1
+        cl = qpso.make_cluster(rect, c["centroid_mm"], npts, c["depth"])
+        with pytest.raises(qpso.QpsoError) as e:
+            ctx.solve(mid + 1000 if case == "bad_mesh" else mid, cl, p)
+        assert e.value.code == (qpso.E_NOTFOUND if case == "bad_mesh" else qpso.E_INVAL)
+    finally:
+        ctx.release_mesh(mid)
