@@ -149,18 +149,38 @@ def test_solve_config1_short(ctx, scene1):
     check_solve(*run_solve_pair(ctx, scene, params))
 
 
-def test_big_mesh_windows(ctx):
+@pytest.mark.parametrize("face_cap", [0, 12345])
+def test_big_mesh_windows(ctx, face_cap):
     # 16k faces, 8.2k unique vertices: more than the LDS vertex cache holds,
-    # so the faces are drawn run by run from projected vertex windows
+    # so the faces are drawn run by run from projected vertex windows; with a
+    # face cap a window can reach past the capped faces' vertices
     mesh = qpso.mesh_uv_ellipsoid(103, 81, 70.0, 50.0, 35.0, 4, 8.0, 16.0, 5)
     rng = np.random.default_rng(77)
     pose = W.gt_pose(rng, xy=60.0, z=(700.0, 900.0), seed=77)
     depth, labels = qpso.scene_render([(mesh, pose)], W.CAM, 1100.0, 2.0, 0.05, 78)
     scene = W.Scene([(mesh, pose)], depth, labels)
-    params = qpso.make_params(24, 3, w=0.7, c1=1.5, c2=1.5, cam=W.CAM, seed=0)
+    params = qpso.make_params(24, 3, w=0.7, c1=1.5, c2=1.5, cam=W.CAM, seed=0, face_cap=face_cap)
     poses = sample_poses(scene, params.n_particles, seed=21, wild=0.3)
     check_eval(ctx, scene, params, poses, frames=False)
     check_eval(ctx, scene, params, poses)
+    check_solve(*run_solve_pair(ctx, scene, params))
+
+
+def test_big_mesh_without_runs(ctx):
+    # the same 16k-face mesh with its faces shuffled: corners of neighbouring
+    # faces are far apart in vertex order, so there are no vertex windows and
+    # every face projects its own corners
+    mesh = qpso.mesh_uv_ellipsoid(103, 81, 70.0, 50.0, 35.0, 4, 8.0, 16.0, 5)
+    F = mesh.size // 9
+    perm = np.random.default_rng(5).permutation(F)
+    mesh = np.ascontiguousarray(mesh.reshape(3, 3, F)[:, :, perm].reshape(-1))
+    rng = np.random.default_rng(77)
+    pose = W.gt_pose(rng, xy=60.0, z=(700.0, 900.0), seed=77)
+    depth, labels = qpso.scene_render([(mesh, pose)], W.CAM, 1100.0, 2.0, 0.05, 78)
+    scene = W.Scene([(mesh, pose)], depth, labels)
+    params = qpso.make_params(16, 2, w=0.7, c1=1.5, c2=1.5, cam=W.CAM, seed=0)
+    poses = sample_poses(scene, params.n_particles, seed=22, wild=0.3)
+    check_eval(ctx, scene, params, poses, frames=False)
     check_solve(*run_solve_pair(ctx, scene, params))
 
 
