@@ -783,9 +783,9 @@ int qpso_solve_eval_poses(qpso_solve* s, int n, const float* poses, float* fit, 
     tmp.dbg_aabb = d_aabb;
     if (hipMalloc(&d_tmp, sizeof(QpsoSolveDev)) != hipSuccess) { cleanup(); return QPSO_E_NOMEM; }
     hipError_t e = hipMemcpyAsync(d_tmp, &tmp, sizeof(tmp), hipMemcpyHostToDevice, st);
-    // frames come from the window instantiation (it handles cached meshes too)
+    // frames come from their own instantiation (it handles every mesh)
     if (e == hipSuccess)
-        e = qpso_launch::render_fitness(s->rf_cfg, needs_windows(s, s->rf_cfg) || frames != nullptr, d_tmp, nullptr, n, st);
+        e = qpso_launch::render_fitness(s->rf_cfg, frames ? 2 : needs_windows(s, s->rf_cfg) ? 1 : 0, d_tmp, nullptr, n, st);
     if (e == hipSuccess && fit) e = hipMemcpyAsync(fit, A + s->L.fit, sizeof(float) * n, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess && aabb) e = hipMemcpyAsync(aabb, d_aabb, sizeof(int32_t) * 4 * n, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess && frames)

@@ -19,10 +19,9 @@ hipError_t init(const QpsoSolveDev* d_solves, int n_solves, hipStream_t st);
 bool rf_needs_windows(int cfg, int v_eff);
 // One fused render+fitness pass; d_items[i] = (solve index, particle index),
 // or d_items = nullptr for workgroup i = particle i of d_solves[0].
-// win: the instantiation that draws meshes beyond the vertex cache window by
-// window (it handles every mesh and the frame dump; the other one only
-// cached meshes without frames).
-hipError_t render_fitness(int cfg, bool win, const QpsoSolveDev* d_solves, const QpsoItem* d_items, int n_items,
+// mode 0: meshes that fit the vertex cache; 1: any mesh (window by window
+// beyond the cache); 2: any mesh with the frame dump of the stage call.
+hipError_t render_fitness(int cfg, int mode, const QpsoSolveDev* d_solves, const QpsoItem* d_items, int n_items,
                           hipStream_t st);
 // argmin + pbest/gbest + particle update, one workgroup per solve.
 hipError_t pso_step(const QpsoSolveDev* d_solves, int n_solves, hipStream_t st);

@@ -142,9 +142,11 @@ struct Visits {
 
 // WIN: the instantiation that can draw a mesh window by window (larger than
 // the vertex cache); the other one keeps the cached path free of it.
-template <int BT, int CAP, bool WIN>
+template <int BT, int CAP, int MODE>
 __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __restrict__ solves,
                                                       const QpsoItem* __restrict__ items) {
+    constexpr bool WIN = MODE >= 1;     // meshes beyond the vertex cache, window by window
+    constexpr bool FRAMES = MODE == 2;  // and the frame dump (stage calls)
     constexpr int SPT = 2048 / BT;  // fitness slots per thread
     __shared__ int32_t lds[CAP];    // [vertex cache X | Y | Z] [depth tile]
     __shared__ int32_t s_box[4];    // rendered AABB: minX, maxX, minY, maxY
@@ -202,8 +204,8 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         }
     }
     if (iter >= S.I) return;  // this solve has finished (batched runs)
-    // launched only for meshes that fit the vertex cache, and without frames
-    if (!WIN && (Ve > kWin || S.dbg_frames)) {
+    // each instantiation only for the meshes and outputs it handles
+    if ((!WIN && Ve > kWin) || (!FRAMES && S.dbg_frames)) {
         if (tid == 0) {
             S.fit[p] = __builtin_nanf("");
             atomicOr(S.flags, QPSO_FLAG_LAUNCH);
@@ -373,7 +375,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     // next iteration: those two lines keep exact depth in their own arrays.
     // A stage call that dumps the rendered frames keeps the depth of the whole
     // tile (D = tile, no mask).
-    const bool full = WIN && S.dbg_frames != nullptr;  // frames: the window instantiation only
+    const bool full = FRAMES && S.dbg_frames != nullptr;
     const int32_t dx0 = full ? tx0 : max(tx0, rx0), dx1 = full ? tx1 : min(tx1, rx1);
     const int32_t dy0 = full ? ty0 : max(ty0, ry0), dy1 = full ? ty1 : min(ty1, ry1);
     const int32_t DW = dx0 < dx1 && dy0 < dy1 ? dx1 - dx0 : 0;
@@ -1181,20 +1183,21 @@ hipError_t init(const QpsoSolveDev* d_solves, int n_solves, hipStream_t st) {
 }
 
 template <int BT, int CAP>
-static void launch_rf(bool win, const QpsoSolveDev* d_solves, const QpsoItem* d_items, int n_items, hipStream_t st) {
-    if (win) hipLaunchKernelGGL((k_render_fitness<BT, CAP, true>), dim3(n_items), dim3(BT), 0, st, d_solves, d_items);
-    else hipLaunchKernelGGL((k_render_fitness<BT, CAP, false>), dim3(n_items), dim3(BT), 0, st, d_solves, d_items);
+static void launch_rf(int mode, const QpsoSolveDev* d_solves, const QpsoItem* d_items, int n_items, hipStream_t st) {
+    if (mode == 2) hipLaunchKernelGGL((k_render_fitness<BT, CAP, 2>), dim3(n_items), dim3(BT), 0, st, d_solves, d_items);
+    else if (mode == 1) hipLaunchKernelGGL((k_render_fitness<BT, CAP, 1>), dim3(n_items), dim3(BT), 0, st, d_solves, d_items);
+    else hipLaunchKernelGGL((k_render_fitness<BT, CAP, 0>), dim3(n_items), dim3(BT), 0, st, d_solves, d_items);
 }
 
-hipError_t render_fitness(int cfg, bool win, const QpsoSolveDev* d_solves, const QpsoItem* d_items, int n_items,
+hipError_t render_fitness(int cfg, int mode, const QpsoSolveDev* d_solves, const QpsoItem* d_items, int n_items,
                           hipStream_t st) {
     if (n_items <= 0) return hipSuccess;
     switch (cfg) {
-        case 0: launch_rf<1024, 40448>(win, d_solves, d_items, n_items, st); break;
-        case 1: launch_rf<512, 40448>(win, d_solves, d_items, n_items, st); break;
-        case 2: launch_rf<512, 20224>(win, d_solves, d_items, n_items, st); break;
-        case 3: launch_rf<256, 20224>(win, d_solves, d_items, n_items, st); break;
-        case 4: launch_rf<256, 10112>(win, d_solves, d_items, n_items, st); break;
+        case 0: launch_rf<1024, 40448>(mode, d_solves, d_items, n_items, st); break;
+        case 1: launch_rf<512, 40448>(mode, d_solves, d_items, n_items, st); break;
+        case 2: launch_rf<512, 20224>(mode, d_solves, d_items, n_items, st); break;
+        case 3: launch_rf<256, 20224>(mode, d_solves, d_items, n_items, st); break;
+        case 4: launch_rf<256, 10112>(mode, d_solves, d_items, n_items, st); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
