@@ -128,15 +128,18 @@ __global__ __launch_bounds__(256) void k_init(const QpsoSolveDev* __restrict__ s
 // offset (xb, k) of the union box: the row index of t is the fp32 product
 // trunc(t * rcp(W)), so for t = kW it falls one row short when that product
 // rounds below k; (0, k) is then skipped by t = kW and visited by t = (k+1)W.
+// __mul24 (the full-rate 24-bit multiply) forms products of pixel
+// coordinates and row lengths: every such operand is below 2^23 in magnitude
+// (the union box is at most 2^23 pixels), so the product is exact.
 struct Visits {
     int32_t W, H, T;
     float rW;
     __device__ __forceinline__ bool short_row(int32_t k) const {
-        return k > 0 && row_of(k * W, rW) == k - 1;
+        return k > 0 && row_of(__mul24(k, W), rW) == k - 1;
     }
     __device__ __forceinline__ int32_t at(int32_t xb, int32_t k) const {
-        if (xb != 0) return (k < H || k * W + xb < T) ? 1 : 0;
-        return ((k * W < T && !short_row(k)) ? 1 : 0) + (((k + 1) * W < T && short_row(k + 1)) ? 1 : 0);
+        if (xb != 0) return (k < H || __mul24(k, W) + xb < T) ? 1 : 0;
+        return ((__mul24(k, W) < T && !short_row(k)) ? 1 : 0) + ((__mul24(k + 1, W) < T && short_row(k + 1)) ? 1 : 0);
     }
 };
 
@@ -412,14 +415,14 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     int32_t db0 = 0;  // first D row of the current band
     auto raster_exact = [&](const Face& fc, const Setup& su, int32_t x0, int32_t x1, int32_t y0, int32_t y1) {
         if (x0 >= x1 || y0 >= y1) return;
-        const int32_t w = x1 - x0, n = w * (y1 - y0);
+        const int32_t w = x1 - x0, n = __mul24(w, y1 - y0);
         int32_t CX2 = wsub(wadd(su.CY2, wmul(y0 - fc.miny, su.FDX23)), wmul(x0 - fc.minx, su.FDY23));
         int32_t CX3 = wsub(wadd(su.CY3, wmul(y0 - fc.miny, su.FDX31)), wmul(x0 - fc.minx, su.FDY31));
         const int32_t step2 = wsub(0, su.FDY23), step3 = wsub(0, su.FDY31);
         const int32_t wrap2 = wadd(su.FDX23, wmul(w - 1, su.FDY23));
         const int32_t wrap3 = wadd(su.FDX31, wmul(w - 1, su.FDY31));
         const int32_t awrap = DW - w + 1;
-        int32_t* addr = tile + (y0 - db0) * DW + (x0 - dx0);
+        int32_t* addr = tile + __mul24(y0 - db0, DW) + (x0 - dx0);
         int32_t c = w;
         for (int32_t m = 0; m < n; ++m) {
             atomicMin(addr, pixel_depth(CX2, CX3, su));
@@ -440,13 +443,13 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         const int32_t x0 = bq_x[q] & 0xffff, x1 = bq_x[q] >> 16;
         const int32_t y0 = bq_y[q] & 0xffff, y1 = bq_y[q] >> 16;
         const int32_t kd = bq_k[q], fi = bq_i[q];
-        const int32_t w = x1 - x0, n = w * (y1 - y0);
+        const int32_t w = x1 - x0, n = __mul24(w, y1 - y0);
         const int32_t lane = tid & 63;
         const float rw = __builtin_amdgcn_rcpf((float)w);
-        const int32_t sy = (int32_t)(64.5f * rw), sx = 64 - sy * w;
-        int32_t dy = (int32_t)(((float)lane + 0.5f) * rw), dx = lane - dy * w;
-        const int32_t astep = sy * DW + sx, afix = DW - w;
-        int32_t off = (y0 + dy - db0) * DW + (x0 + dx - dx0);
+        const int32_t sy = (int32_t)(64.5f * rw), sx = 64 - __mul24(sy, w);
+        int32_t dy = (int32_t)(((float)lane + 0.5f) * rw), dx = lane - __mul24(dy, w);
+        const int32_t astep = __mul24(sy, DW) + sx, afix = DW - w;
+        int32_t off = __mul24(y0 + dy - db0, DW) + (x0 + dx - dx0);
         if (fi & 1) {  // flat: one depth over the rectangle
             for (int32_t m = lane; m < n; m += 64) {
                 atomicMin(tile + off, kd);
@@ -496,7 +499,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
                 int32_t o[U], xs[U], ys[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    o[u] = i + u * BT < n ? (int32_t)obs[(y - ry0) * rw + (x - rx0)] : 0;
+                    o[u] = i + u * BT < n ? (int32_t)obs[__mul24(y - ry0, rw) + (x - rx0)] : 0;
                     xs[u] = x; ys[u] = y;
                     x += dr; y += dq;
                     if (x >= x1) { x -= sw; ++y; }
@@ -580,7 +583,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             const int32_t w0 = (x0 - tx0) >> 5, w1 = (x1 - 1 - tx0) >> 5;
             const uint32_t m0 = ~0u << ((x0 - tx0) & 31), m1 = ~0u >> (31 - ((x1 - 1 - tx0) & 31));
             for (int32_t y = y0; y < y1; ++y) {
-                uint32_t* row = mask + (y - b0) * MW;
+                uint32_t* row = mask + __mul24(y - b0, MW);
                 for (int32_t w = w0; w <= w1; ++w) {
                     const uint32_t m = (w == w0 ? m0 : ~0u) & (w == w1 ? m1 : ~0u);
                     atomicOr(&row[w], m);
@@ -638,7 +641,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
                 for (int32_t x = x0; x < x1; ++x) atomicMin(&row_d[x - tx0], one ? kmin : depth_at(fc, su, x, maxY));
             }
             if (!in_d) return;
-            const int32_t w = px1 - px0, n = w * (py1 - py0);
+            const int32_t w = px1 - px0, n = __mul24(w, py1 - py0);
             int32_t qb;
             // a non-flat face costs a per-pixel loop in its lane, so it goes to
             // the wave queue from half the threshold of a flat one
@@ -650,7 +653,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             } else if (flat) {
                 // rows in chunks of four pixels; a chunk's positions past the
                 // row's end repeat its last pixel (the same minimum again)
-                int32_t* row = tile + (py0 - db0) * DW + (px0 - dx0);
+                int32_t* row = tile + __mul24(py0 - db0, DW) + (px0 - dx0);
                 const int32_t last = w - 1;
                 int32_t c = 0, rows = py1 - py0;
                 while (rows > 0) {
@@ -839,12 +842,12 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
                         zk[u][j] = 0;
                         if (t < tlo || t >= thi) continue;
                         const int32_t y = row_of(t, vis.rW);
-                        int32_t x = t - y * W;
+                        int32_t x = t - __mul24(y, W);
                         if (x >= W) x -= W;  // t = kW mapped to row k-1
                         const int32_t px = minX + x, py = minY + y;
                         if (px < qx0 || px >= qx1 || py < qy0 || py >= qy1) continue;
-                        zr[u][j] = tile[(py - db0) * DW + px - dx0];
-                        zk[u][j] = (int32_t)obs[(py - ry0) * rw + (px - rx0)];
+                        zr[u][j] = tile[__mul24(py - db0, DW) + px - dx0];
+                        zk[u][j] = (int32_t)obs[__mul24(py - ry0, rw) + (px - rx0)];
                     }
                 }
 #pragma unroll
