@@ -151,7 +151,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     constexpr bool WIN = MODE >= 1;     // meshes beyond the vertex cache, window by window
     constexpr bool FRAMES = MODE == 2;  // and the frame dump (stage calls)
     constexpr int SPT = 2048 / BT;  // fitness slots per thread
-    __shared__ int32_t lds[CAP];    // [vertex cache X | Y | Z] [depth tile]
+    __shared__ __attribute__((aligned(16))) int32_t lds[CAP];    // [vertex cache: (X, Y) pairs, Z] [depth tile]
     __shared__ int32_t s_box[4];    // rendered AABB: minX, maxX, minY, maxY
     __shared__ int32_t s_stale[4];  // carried pixels inside the union box: x0, x1, y0, y1 (inclusive)
     __shared__ int32_t s_degen;
@@ -242,8 +242,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     const bool cached = !WIN || Ve <= kWin;  // compile-time true without windows
     const bool runs = WIN && !cached && S.n_runs > 0 && S.n_runs * 256 <= Fe;  // runs of 256 faces on average
     const int VC = cached ? Ve : (runs ? kWin : 0);  // cache capacity in vertices
-    int32_t* __restrict__ vX = lds;
-    int32_t* __restrict__ vY = lds + VC;
+    int2* __restrict__ vXY = reinterpret_cast<int2*>(lds);  // X, Y of a vertex in one 8-byte read
     float* __restrict__ vZ = reinterpret_cast<float*>(lds + 2 * VC);
     int32_t vbase = 0;  // vertex index of cache entry 0
     constexpr int kQCap = CAP / 32;   // small faces needing the per-pixel depth
@@ -281,7 +280,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
                 int32_t X, Y;
                 float Z;
                 dg |= vertex_fixed(pose, x, y, z, cam, X, Y, Z) ? 1 : 0;
-                if (cached) { vX[v] = X; vY[v] = Y; vZ[v] = Z; }
+                if (cached) { vXY[v] = make_int2(X, Y); vZ[v] = Z; }
                 xl = min(xl, X); xh = max(xh, X);
                 yl = min(yl, Y); yh = max(yh, Y);
             };
@@ -401,9 +400,10 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         if (!WIN || VC > 0) {  // always cached without windows
             const int32_t vb = WIN ? vbase : 0;
             const int32_t a = e.v0 - vb, b = e.v1 - vb, c = e.v2 - vb;
-            fc.X[0] = vX[a]; fc.Y[0] = vY[a]; fc.Z[0] = vZ[a];
-            fc.X[1] = vX[b]; fc.Y[1] = vY[b]; fc.Z[1] = vZ[b];
-            fc.X[2] = vX[c]; fc.Y[2] = vY[c]; fc.Z[2] = vZ[c];
+            const int2 pa = vXY[a], pb = vXY[b], pc = vXY[c];
+            fc.X[0] = pa.x; fc.Y[0] = pa.y; fc.Z[0] = vZ[a];
+            fc.X[1] = pb.x; fc.Y[1] = pb.y; fc.Z[1] = vZ[b];
+            fc.X[2] = pc.x; fc.Y[2] = pc.y; fc.Z[2] = vZ[c];
             face_bbox(fc);
         } else {
             face_project(mesh, F, e.face, pose, cam, fc);
@@ -700,7 +700,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
                 int32_t X, Y;
                 float Z;
                 vertex_fixed(pose, wx[v], wy[v], wz[v], cam, X, Y, Z);
-                vX[v - vlo] = X; vY[v - vlo] = Y; vZ[v - vlo] = Z;
+                vXY[v - vlo] = make_int2(X, Y); vZ[v - vlo] = Z;
             }
             __syncthreads();
         }

@@ -321,11 +321,11 @@ QD bool edge_range64(int32_t CY, int32_t FDX, int32_t FDY, int32_t ia, int32_t i
 
 // The same in 32 bits, for |FDX|, |FDY| < 2^18, |CY| < 2^28 and offsets below
 // 1024: every product is below 2^28 in magnitude and every corner value below
-// 5 * 2^28, so nothing wraps.
+// 5 * 2^28, so nothing wraps; the operands fit the 24-bit multiply.
 QD void edge_range32(int32_t CY, int32_t FDX, int32_t FDY, int32_t ia, int32_t ib, int32_t ja, int32_t jb,
                      int32_t& lo, int32_t& hi) {
-    const int32_t e = CY + FDX * ja - FDY * ia;
-    const int32_t dx = -FDY * (ib - ia), dy = FDX * (jb - ja);
+    const int32_t e = CY + __mul24(FDX, ja) - __mul24(FDY, ia);
+    const int32_t dx = -__mul24(FDY, ib - ia), dy = __mul24(FDX, jb - ja);
     lo = (e + min(dx, 0) + min(dy, 0) + 15) >> 8;
     hi = (e + max(dx, 0) + max(dy, 0) + 15) >> 8;
 }
