@@ -234,35 +234,34 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     }
     __syncthreads();
 
-    // The projected unique vertices stay in LDS when they take at most half of
-    // it (kWin vertices).  A larger mesh is cut on the host into runs of
-    // faces whose corners lie in a window of at most kWin vertex indices;
-    // each band then projects the windows in turn.  Without usable runs every
-    // face's corners are projected where needed.
+    // The projected unique vertices of a mesh of at most kWin vertices stay in
+    // LDS.  A larger mesh is cut on the host into runs of faces whose corners
+    // lie in a window of at most kWin vertex indices; each band then projects
+    // the windows in turn.  Without usable runs every face's corners are
+    // projected where needed.
+    //   A vertex takes three words, X and Y read together, then Z; or, packed,
+    // one word, X and Y as 16-bit halves, and a drawn face's corner depths
+    // are projected again from the face's mesh coordinates.  Packing gives the
+    // tile two words per vertex for that re-projection; it is taken (meshes
+    // without windows) when a depth region covering the whole cluster rect
+    // would not fit beside the three-word cache in one band and every 28.4
+    // coordinate of the particle fits the halves (checked after pass A).
     const bool cached = !WIN || Ve <= kWin;  // compile-time true without windows
     const bool runs = WIN && !cached && S.n_runs > 0 && S.n_runs * 256 <= Fe;  // runs of 256 faces on average
     const int VC = cached ? Ve : (runs ? kWin : 0);  // cache capacity in vertices
-    int2* __restrict__ vXY = reinterpret_cast<int2*>(lds);  // X, Y of a vertex in one 8-byte read
+    constexpr int kQCap = CAP / 32;   // small faces needing the per-pixel depth
+    constexpr int kBQCap = CAP / 128;
+    constexpr int kRing = 128;
+    constexpr int kFixed = kQCap + 4 * kBQCap + (BT / 64) * kRing + 4;  // queues, rings, alignment
+    bool pack = false;
+    if (!WIN) {
+        const int32_t rw = S.rect[2] - S.rect[0], rh = S.rect[3] - S.rect[1];
+        pack = rw * rh + rh * (((rw + 31) >> 5) + 1) + rw + 8 > CAP - kFixed - 3 * VC;
+    }
+    uint32_t* __restrict__ vP = reinterpret_cast<uint32_t*>(lds);  // packed: X | Y << 16
+    int2* __restrict__ vXY = reinterpret_cast<int2*>(lds);  // X, Y in one 8-byte read
     float* __restrict__ vZ = reinterpret_cast<float*>(lds + 2 * VC);
     int32_t vbase = 0;  // vertex index of cache entry 0
-    constexpr int kQCap = CAP / 32;   // small faces needing the per-pixel depth
-    // faces of more than S.big_face pixels (non-flat ones: half as many),
-    // rasterized by a whole wave so one large face does not hold its wave's
-    // other 63 lanes idle: clipped
-    // rectangle (x0 | x1 << 16, y0 | y1 << 16), constant depth, face << 1 | flat
-    constexpr int kBQCap = CAP / 128;
-    int32_t* __restrict__ queue = lds + 3 * VC;
-    int32_t* __restrict__ bq_x = queue + kQCap;
-    int32_t* __restrict__ bq_y = bq_x + kBQCap;
-    int32_t* __restrict__ bq_k = bq_y + kBQCap;
-    int32_t* __restrict__ bq_i = bq_k + kBQCap;
-    // per wave: ring of culled face entries waiting to be drawn
-    constexpr int kRing = 128;
-    int32_t* __restrict__ rings = bq_i + kBQCap;
-    // the tile starts on a 16-byte boundary for 128-bit fills
-    const int32_t tile_off = (int32_t)(rings - lds) + (BT / 64) * kRing;
-    int32_t* __restrict__ tile = lds + ((tile_off + 3) & ~3);
-    const int32_t tcap = CAP - ((tile_off + 3) & ~3);
 
     // ---- pass A: per-face bbox and AABB over every face (main.h:1340-1353,
     //      AABBreduction main.h:1190-1304, final reduction main.cu:423-546).
@@ -280,7 +279,8 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
                 int32_t X, Y;
                 float Z;
                 dg |= vertex_fixed(pose, x, y, z, cam, X, Y, Z) ? 1 : 0;
-                if (cached) { vXY[v] = make_int2(X, Y); vZ[v] = Z; }
+                if (pack) vP[v] = ((uint32_t)X & 0xffffu) | ((uint32_t)Y << 16);  // used only if all fit
+                else if (cached) { vXY[v] = make_int2(X, Y); vZ[v] = Z; }
                 xl = min(xl, X); xh = max(xh, X);
                 yl = min(yl, Y); yh = max(yh, Y);
             };
@@ -327,6 +327,41 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         }
         return;
     }
+
+    // (X + 15) >> 4 >= -2047 gives X >= -32767 and <= 2046 gives X <= 32736:
+    // every cached vertex fits the 16-bit halves.  A particle with vertices
+    // beyond them writes the three-word cache instead (its vertices are still
+    // in registers; the packed words are never read) and takes more bands.
+    if (pack && !(ax0 >= -2047 && ax1 <= 2046 && ay0 >= -2047 && ay1 <= 2046)) {
+        pack = false;
+#pragma unroll
+        for (int j = 0; j < kVPT; ++j) {
+            const int v = tid + j * BT;
+            if (v < Ve) {
+                int32_t X, Y;
+                float Z;
+                vertex_fixed(pose, ux[j], uy[j], uz[j], cam, X, Y, Z);
+                vXY[v] = make_int2(X, Y);
+                vZ[v] = Z;
+            }
+        }
+    }
+    // queue of small faces needing the per-pixel depth (kQCap entries); big
+    // faces of more than S.big_face pixels (non-flat ones: half as many),
+    // rasterized by a whole wave so one large face does not hold its wave's
+    // other 63 lanes idle: clipped
+    // rectangle (x0 | x1 << 16, y0 | y1 << 16), constant depth, face << 1 | flat
+    int32_t* __restrict__ queue = lds + (pack ? VC : 3 * VC);
+    int32_t* __restrict__ bq_x = queue + kQCap;
+    int32_t* __restrict__ bq_y = bq_x + kBQCap;
+    int32_t* __restrict__ bq_k = bq_y + kBQCap;
+    int32_t* __restrict__ bq_i = bq_k + kBQCap;
+    // per wave: ring of kRing culled face entries waiting to be drawn
+    int32_t* __restrict__ rings = bq_i + kBQCap;
+    // the tile starts on a 16-byte boundary for 128-bit fills
+    const int32_t tile_off = (int32_t)(rings - lds) + (BT / 64) * kRing;
+    int32_t* __restrict__ tile = lds + ((tile_off + 3) & ~3);
+    const int32_t tcap = CAP - ((tile_off + 3) & ~3);
 
     const int32_t W = (int32_t)W64, H = (int32_t)H64;
     const int32_t A = W * H;
@@ -395,9 +430,25 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         if (k < cap) store_stale(nw, k, (y << 16) | x, v);
         else atomicOr(S.flags, QPSO_FLAG_STALE_OVERFLOW);
     };
-    // the face's 28.4 vertices: from the LDS vertex cache, or projected again
-    auto load_face = [&](const QpsoFaceRef& e, Face& fc) {
-        if (!WIN || VC > 0) {  // always cached without windows
+    // the face's 28.4 vertices: from the LDS vertex cache, or projected again.
+    // f is the face's index, e its table entry (the host keeps the table in
+    // face order, e.face == f; passing f lets the mesh reads go out without
+    // waiting for the entry)
+    auto load_face = [&](int32_t f, const QpsoFaceRef& e, Face& fc) {
+        if (!WIN && pack) {
+            const uint32_t pa = vP[e.v0], pb = vP[e.v1], pc = vP[e.v2];
+            fc.X[0] = (int32_t)(pa << 16) >> 16; fc.Y[0] = (int32_t)pa >> 16;
+            fc.X[1] = (int32_t)(pb << 16) >> 16; fc.Y[1] = (int32_t)pb >> 16;
+            fc.X[2] = (int32_t)(pc << 16) >> 16; fc.Y[2] = (int32_t)pc >> 16;
+            // corner depths: pass A's projection of the same coordinates
+            const QG float* __restrict__ m = (const QG float*)mesh + f;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                float u, v;
+                project(pose, m[j * F], m[3 * F + j * F], m[6 * F + j * F], cam, u, v, fc.Z[j]);
+            }
+            face_bbox(fc);
+        } else if (!WIN || VC > 0) {  // always cached without windows
             const int32_t vb = WIN ? vbase : 0;
             const int32_t a = e.v0 - vb, b = e.v1 - vb, c = e.v2 - vb;
             const int2 pa = vXY[a], pb = vXY[b], pc = vXY[c];
@@ -461,7 +512,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         }
         Face fc;
         Setup su;
-        load_face(load_ref(faces, fi >> 1), fc);
+        load_face(fi >> 1, load_ref(faces, fi >> 1), fc);
         face_setup(fc, su);
         int32_t CX2 = wsub(wadd(su.CY2, wmul(y0 + dy - fc.miny, su.FDX23)), wmul(x0 + dx - fc.minx, su.FDY23));
         int32_t CX3 = wsub(wadd(su.CY3, wmul(y0 + dy - fc.miny, su.FDX31)), wmul(x0 + dx - fc.minx, su.FDY31));
@@ -607,7 +658,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         auto draw = [&](int32_t i) {
             Face fc;
             Setup su;
-            load_face(load_ref(faces, i), fc);
+            load_face(i, load_ref(faces, i), fc);
             face_setup(fc, su);
             const int32_t x0 = max(fc.minx, tx0), x1 = min(fc.maxx, tx1);
             const int32_t y0 = max(fc.miny, b0), y1 = min(fc.maxy, b1);
@@ -717,7 +768,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
                 bool later = false;
                 if (e.face < Fe) {
                     Face fc;
-                    load_face(e, fc);
+                    load_face(i, e, fc);
                     if (max(fc.minx, tx0) < min(fc.maxx, tx1) && fc.miny < ty1 && fc.maxy > b0 && face_front(fc)) {
                         later = fc.maxy > b1;
                         keep = fc.miny < b1;
@@ -743,7 +794,8 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         for (int q = tid; q < nq; q += BT) {
             Face fc;
             Setup su;
-            load_face(load_ref(faces, queue[q]), fc);
+            const int32_t f = queue[q];
+            load_face(f, load_ref(faces, f), fc);
             face_setup(fc, su);
             raster_exact(fc, su, max(fc.minx, dx0), min(fc.maxx, dx1), max(fc.miny, db0), min(fc.maxy, db1));
         }

@@ -184,6 +184,26 @@ def test_big_mesh_without_runs(ctx):
     check_solve(*run_solve_pair(ctx, scene, params))
 
 
+def test_packed_cache_far_vertices(ctx, scene1):
+    # the 201x133 cluster rect takes the packed vertex cache (16-bit X, Y);
+    # a pose close to the camera and off-axis puts vertices beyond 2047
+    # pixels, so that particle projects its faces where needed instead
+    scene, params = scene1
+    poses = sample_poses(scene, params.n_particles, seed=31)
+    rng = np.random.default_rng(31)
+    gt = scene.objects[0][1]
+    for i in range(8):
+        q = gt[3:] + rng.normal(0, 0.05, 4)
+        q = q / np.linalg.norm(q)
+        poses[:, 5 * i] = [0.2 + 0.01 * i, rng.uniform(-0.02, 0.02), 0.15 + 0.005 * i, *q]
+    poses[:, 3] = [0.3, 0.0, 0.2, *gt[3:]]
+    fit = check_eval(ctx, scene, params, poses, frames=False)
+    far = [i for i in range(0, 40, 5)]
+    fo, ao, _ = oracle_eval(scene, params, poses[:, far])
+    assert (ao[:, 1] > 2046).any() and np.isfinite(fo[ao[:, 1] > 2046]).any()
+    assert np.isfinite(fit).sum() > params.n_particles // 2
+
+
 @pytest.mark.parametrize("cfg", [1, 2, 3, 4])
 def test_tile_configs_agree(scene1, cfg):
     # every (threads, tile) variant of the render+fitness kernel is bit-exact
