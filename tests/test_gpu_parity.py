@@ -149,6 +149,15 @@ def test_solve_config1_short(ctx, scene1):
     check_solve(*run_solve_pair(ctx, scene, params))
 
 
+def test_solve_config1_face_cap(ctx, scene1):
+    # the reference's 3,072-face cap on the packed vertex cache: only the
+    # vertices of the first 3,072 faces are cached and drawn
+    scene, params = scene1
+    p = qpso.make_params(params.n_particles, params.n_iters, w=0.7, c1=1.5, c2=1.5, cam=W.CAM, seed=3,
+                         face_cap=3072)
+    check_solve(*run_solve_pair(ctx, scene, p))
+
+
 @pytest.mark.parametrize("face_cap", [0, 12345])
 def test_big_mesh_windows(ctx, face_cap):
     # 16k faces, 8.2k unique vertices: more than the LDS vertex cache holds,
