@@ -174,7 +174,7 @@ void build_mesh_tables(const float* A, uint32_t F, std::vector<float>& verts, st
 }
 
 struct ArenaLayout {
-    size_t desc, obs, pos, vel, pbest, pbest_fit, fit, gbest, iter, flags, counters;
+    size_t desc, obs, obs_rows, pos, vel, pbest, pbest_fit, fit, gbest, iter, flags, counters;
     size_t stale0, stale1, cnt0, cnt1, fit_tr, gbest_tr, argmin_tr, total;
 };
 
@@ -184,6 +184,7 @@ ArenaLayout layout(int N, int I, int rw, int rh, bool trace) {
     auto take = [&](size_t bytes) { size_t r = o; o = align_up(o + bytes, 256); return r; };
     L.desc = take(sizeof(QpsoSolveDev));
     L.obs = take(sizeof(uint16_t) * (size_t)rw * rh);
+    L.obs_rows = take(sizeof(uint16_t) * (size_t)(rw + 1) * rh);
     L.pos = take(sizeof(float) * 7 * N);
     L.vel = take(sizeof(float) * 7 * N);
     L.pbest = take(sizeof(float) * 7 * N);
@@ -535,6 +536,7 @@ int qpso_solve_create(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl, c
     std::memcpy(h.rect, cl->rect, sizeof(h.rect));
     h.num_points = cl->num_points;
     h.obs = reinterpret_cast<const uint16_t*>(A + s->L.obs);
+    h.obs_rows = reinterpret_cast<const uint16_t*>(A + s->L.obs_rows);
     h.rand = rand;
     h.pos = reinterpret_cast<float*>(A + s->L.pos);
     h.vel = reinterpret_cast<float*>(A + s->L.vel);
@@ -564,9 +566,17 @@ int qpso_solve_create(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl, c
     for (int y = 0; y < rh; ++y)
         std::memcpy(&crop[(size_t)y * rw], cl->depth + (size_t)(cl->rect[1] + y) * kCols + cl->rect[0],
                     sizeof(uint16_t) * rw);
+    // per crop row, the number of observed (non-zero) pixels left of each column
+    std::vector<uint16_t> rows((size_t)(rw + 1) * rh);
+    for (int y = 0; y < rh; ++y) {
+        uint16_t* r = &rows[(size_t)y * (rw + 1)];
+        r[0] = 0;
+        for (int x = 0; x < rw; ++x) r[x + 1] = (uint16_t)(r[x] + (crop[(size_t)y * rw + x] != 0 ? 1 : 0));
+    }
     hipStream_t st = ctx->stream;
     int rc = QPSO_OK;
     if (hipMemcpyAsync(A + s->L.obs, crop.data(), crop.size() * 2, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(A + s->L.obs_rows, rows.data(), rows.size() * 2, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipMemcpyAsync(s->d_desc, &s->h, sizeof(QpsoSolveDev), hipMemcpyHostToDevice, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
         rc = set_err(ctx, QPSO_E_HIP, "solve upload failed");

@@ -65,6 +65,7 @@ struct QpsoSolveDev {
     int32_t rect[4];        // minx, miny, maxx, maxy (max exclusive)
     uint32_t num_points;
     const uint16_t* obs;    // rect crop of the cluster depth, pitch rect[2]-rect[0]
+    const uint16_t* obs_rows;  // [rect rows][pitch + 1]: observed pixels of the crop row left of each column
     // state
     const float* rand;      // [S][N]
     float* pos;             // [7][N]

@@ -538,26 +538,21 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     // ---- cluster pixels outside the tile region: depth buffer reads 60000
     //      there, so only "observed but not rendered" counts (order-free).
     //      The rect minus the tile is at most four strips: rows above and
-    //      below the tile, and left and right of it within its rows.
+    //      below the tile, and left and right of it within its rows.  Rect
+    //      rows lie above the union box's last row, so every pixel there is
+    //      visited once except those of the column x = minX (vis.at(0, k)
+    //      times): a strip row adds its observed pixels, counted from the
+    //      crop's per-row prefix table, and corrects that column.
     {
+        const QG uint16_t* __restrict__ obs_rows = (const QG uint16_t*)S.obs_rows;
         auto strip = [&](int32_t x0, int32_t x1, int32_t y0, int32_t y1) {
             if (x0 >= x1 || y0 >= y1) return;
-            const int32_t sw = x1 - x0, n = sw * (y1 - y0);
-            const int32_t dq = BT / sw, dr = BT % sw;
-            int32_t y = y0 + tid / sw, x = x0 + tid % sw;
-            constexpr int U = 4;  // loads in flight per thread
-            for (int i = tid; i < n; i += U * BT) {
-                int32_t o[U], xs[U], ys[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    o[u] = i + u * BT < n ? (int32_t)obs[__mul24(y - ry0, rw) + (x - rx0)] : 0;
-                    xs[u] = x; ys[u] = y;
-                    x += dr; y += dq;
-                    if (x >= x1) { x -= sw; ++y; }
-                }
-#pragma unroll
-                for (int u = 0; u < U; ++u)
-                    if (o[u] != 0) n_seg += (uint32_t)vis.at(xs[u] - minX, ys[u] - minY);
+            const bool col = minX >= x0 && minX < x1;
+            for (int32_t y = y0 + tid; y < y1; y += BT) {
+                const QG uint16_t* __restrict__ pr = obs_rows + __mul24(y - ry0, rw + 1);
+                uint32_t c = (uint32_t)pr[x1 - rx0] - (uint32_t)pr[x0 - rx0];
+                if (col && obs[__mul24(y - ry0, rw) + (minX - rx0)] != 0) c = c - 1 + (uint32_t)vis.at(0, y - minY);
+                n_seg += c;
             }
         };
         if (has_tile) {
