@@ -71,7 +71,9 @@ typedef struct {
     int32_t rect[4];       /* rect_aabb_: tl.x, tl.y, br.x, br.y (br exclusive) */
     float centroid_mm[3];  /* centroid3D (mm) */
     uint32_t num_points;   /* num_points */
-    const uint16_t* depth; /* clusterDepth, 480x640, zero outside the cluster */
+    const uint16_t* depth; /* clusterDepth, 480x640, zero outside the cluster;
+                              only the rect is read (pixels outside it count
+                              as unobserved, as they are in a cluster depth) */
 } qpso_cluster;
 
 typedef struct {

@@ -109,6 +109,37 @@ def test_eval_poses_mask_path(ctx, scene0, scene1, wild):
         check_eval(ctx, scene, params, poses, frames=False)
 
 
+def test_rect_strips_quirk_width(ctx, scene0):
+    # a 121-px cluster rect (a width whose fp32 row index skips and doubles
+    # pixels of the union box's first column) over the scene depth inside it
+    # (a cluster's depth is zero outside its rect), so nearly every rect pixel
+    # is observed: rect rows above and below the rendered tile are counted
+    # per row and the x = minX column by its visits
+    scene, params = scene0
+    c = scene.clusters[0]
+    cx, cy = (c["rect"][0] + c["rect"][2]) // 2, (c["rect"][1] + c["rect"][3]) // 2
+    rect = [cx - 60, cy - 55, cx + 61, cy + 55]
+    depth = np.zeros_like(np.asarray(scene.depth, np.uint16))
+    depth[rect[1]:rect[3], rect[0]:rect[2]] = np.asarray(scene.depth, np.uint16)[rect[1]:rect[3], rect[0]:rect[2]]
+    num_points = (rect[2] - rect[0]) * (rect[3] - rect[1])
+    poses = sample_poses(scene, params.n_particles, seed=41, spread_mm=10.0, wild=0.0)
+    mid = ctx.upload_mesh(scene.mesh_m())
+    try:
+        with ctx.solve(mid, qpso.make_cluster(rect, c["centroid_mm"], num_points, depth), params) as s:
+            fit, aabb, _ = s.eval_poses(poses, frames=False)
+    finally:
+        ctx.release_mesh(mid)
+    cam = (params.cam.fx, params.cam.fy, params.cam.cx, params.cam.cy)
+    ofit = []
+    for i in range(poses.shape[1]):
+        fr, info = O.render(scene.mesh_m(), poses[:, i], cam, rect, face_cap=params.face_cap)
+        ofit.append(O.fitness(info, rect, num_points, fr, depth)[0])
+        a = list(info.aabb)
+        assert a[0] >= rect[0] and a[1] <= rect[2] and (a[2] > rect[1] or a[3] < rect[3] - 1)
+    assert_same_f32(fit, np.array(ofit, np.float32), "fitness")
+    assert np.isfinite(fit).all()
+
+
 def run_solve_pair(ctx, scene, params, threads=8):
     mid = ctx.upload_mesh(scene.mesh_m())
     try:
