@@ -73,6 +73,7 @@ struct Mesh {
     float* d = nullptr;            // loadOBJ3 SoA [9F]
     float* verts = nullptr;        // unique vertices [3][V]
     QpsoFaceRef* faces = nullptr;  // interleaved raster order [F]
+    float* corners = nullptr;      // [F][3][4]: a face's corners (x, y, z, 0), one 16-byte load each
     uint32_t F = 0, V = 0;
     std::vector<uint32_t> v_prefix;  // unique vertices referenced by faces [0, f+1)
     int32_t* runs = nullptr;         // (first face, first vertex, end vertex) per run
@@ -85,6 +86,7 @@ void free_mesh(Mesh& m) {
     hipFree(m.d);
     hipFree(m.verts);
     hipFree(m.faces);
+    hipFree(m.corners);
     hipFree(m.runs);
     m = Mesh{};
 }
@@ -448,12 +450,19 @@ int qpso_mesh_upload(qpso_ctx* ctx, const float* soa, uint32_t n_faces, uint32_t
     build_mesh_tables(soa, n_faces, verts, order, m.v_prefix, runs);
     m.n_runs = (int)(runs.size() / 3);
     m.V = (uint32_t)(verts.size() / 3);
+    std::vector<float> corners((size_t)n_faces * 12, 0.0f);
+    for (size_t f = 0; f < n_faces; ++f)
+        for (int j = 0; j < 3; ++j)
+            for (int c = 0; c < 3; ++c) corners[f * 12 + j * 4 + c] = soa[(size_t)c * 3 * n_faces + f + (size_t)j * n_faces];
     hipError_t e = hipMalloc(&m.d, sizeof(float) * 9 * (size_t)n_faces);
+    if (e == hipSuccess) e = hipMalloc(&m.corners, sizeof(float) * corners.size());
     if (e == hipSuccess) e = hipMalloc(&m.verts, sizeof(float) * verts.size());
     if (e == hipSuccess) e = hipMalloc(&m.faces, sizeof(QpsoFaceRef) * order.size());
     if (e == hipSuccess) e = hipMalloc(&m.runs, sizeof(int32_t) * std::max<size_t>(runs.size(), 1));
     if (e == hipSuccess) e = hipMemcpy(m.d, soa, sizeof(float) * 9 * (size_t)n_faces, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(m.verts, verts.data(), sizeof(float) * verts.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(m.corners, corners.data(), sizeof(float) * corners.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(m.faces, order.data(), sizeof(QpsoFaceRef) * order.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !runs.empty())
@@ -521,6 +530,7 @@ int qpso_solve_create(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl, c
     h.V = (int32_t)mit->second.V;
     h.V_eff = (int32_t)mit->second.v_prefix[h.F_eff - 1];
     h.faces = mit->second.faces;
+    h.corners = mit->second.corners;
     h.n_faces = (int32_t)mit->second.F;
     h.runs = mit->second.runs;
     h.n_runs = mit->second.n_runs;

@@ -440,12 +440,15 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             fc.X[0] = (int32_t)(pa << 16) >> 16; fc.Y[0] = (int32_t)pa >> 16;
             fc.X[1] = (int32_t)(pb << 16) >> 16; fc.Y[1] = (int32_t)pb >> 16;
             fc.X[2] = (int32_t)(pc << 16) >> 16; fc.Y[2] = (int32_t)pc >> 16;
-            // corner depths: pass A's projection of the same coordinates
-            const QG float* __restrict__ m = (const QG float*)mesh + f;
+            // corner depths: pass A's projection of the same coordinates,
+            // read as one 16-byte entry per corner
+            typedef float f4 __attribute__((ext_vector_type(4)));
+            const QG f4* __restrict__ cr = (const QG f4*)S.corners + 3 * (size_t)f;
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
+                const f4 c = cr[j];
                 float u, v;
-                project(pose, m[j * F], m[3 * F + j * F], m[6 * F + j * F], cam, u, v, fc.Z[j]);
+                project(pose, c.x, c.y, c.z, cam, u, v, fc.Z[j]);
             }
             face_bbox(fc);
         } else if (!WIN || VC > 0) {  // always cached without windows
