@@ -217,8 +217,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     }
 
     const Cam cam{S.fx, S.fy, S.cx, S.cy};
-    const float* __restrict__ mesh = S.mesh;
-    const int F = S.F, Fe = S.F_eff;
+    const int Fe = S.F_eff;
     const QG QpsoFaceRef* __restrict__ faces = (const QG QpsoFaceRef*)S.faces;
     const int n_faces = S.n_faces;
     const int32_t big_face = S.big_face;
@@ -241,7 +240,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     // projected where needed.
     //   A vertex takes three words, X and Y read together, then Z; or, packed,
     // one word, X and Y as 16-bit halves, and a drawn face's corner depths
-    // are projected again from the face's mesh coordinates.  Packing gives the
+    // are projected again from the face's corner table.  Packing gives the
     // tile two words per vertex for that re-projection; it is taken (meshes
     // without windows) when a depth region covering the whole cluster rect
     // would not fit beside the three-word cache in one band and every 28.4
@@ -432,7 +431,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     };
     // the face's 28.4 vertices: from the LDS vertex cache, or projected again.
     // f is the face's index, e its table entry (the host keeps the table in
-    // face order, e.face == f; passing f lets the mesh reads go out without
+    // face order, e.face == f; passing f lets the corner reads go out without
     // waiting for the entry)
     auto load_face = [&](int32_t f, const QpsoFaceRef& e, Face& fc) {
         if (!WIN && pack) {
@@ -460,7 +459,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             fc.X[2] = pc.x; fc.Y[2] = pc.y; fc.Z[2] = vZ[c];
             face_bbox(fc);
         } else {
-            face_project(mesh, F, e.face, pose, cam, fc);
+            face_project(S.corners, e.face, pose, cam, fc);
         }
     };
     // per-pixel depth over [x0, x1) x [y0, y1): edge values at (x0, y0), per

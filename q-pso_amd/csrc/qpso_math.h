@@ -236,16 +236,17 @@ QD bool vertex_fixed(const float pose[7], float vx, float vy, float vz, const Ca
     return (X > QPSO_FXP_LIMIT) | (X < -QPSO_FXP_LIMIT) | (Y > QPSO_FXP_LIMIT) | (Y < -QPSO_FXP_LIMIT);
 }
 
-QD void face_project(const float* __restrict__ mesh, int F, int f, const float pose[7],
-                     const Cam& c, Face& fc) {
+// Face f projected from the corner table (corner j of face f at
+// corners[12 f + 4 j], (x, y, z, 0): one 16-byte load per corner).
+QD void face_project(const float* __restrict__ corners, int f, const float pose[7], const Cam& c, Face& fc) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const f4* __restrict__ cr = reinterpret_cast<const f4*>(corners) + 3 * (size_t)f;
     fc.degen = false;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-        const float vx = mesh[f + j * F];
-        const float vy = mesh[3 * F + f + j * F];
-        const float vz = mesh[6 * F + f + j * F];
+        const f4 w = cr[j];
         float u, v, Z;
-        project(pose, vx, vy, vz, c, u, v, Z);
+        project(pose, w.x, w.y, w.z, c, u, v, Z);
         fc.X[j] = rint_sat(16.0f * u);
         fc.Y[j] = rint_sat(16.0f * v);
         fc.Z[j] = Z;
