@@ -73,7 +73,7 @@ struct Mesh {
     float* d = nullptr;            // loadOBJ3 SoA [9F]
     float* verts = nullptr;        // unique vertices [3][V]
     QpsoFaceRef* faces = nullptr;  // interleaved raster order [F]
-    float* corners = nullptr;      // [F][3][4]: a face's corners (x, y, z, 0), one 16-byte load each
+    float* corners = nullptr;      // [F][3][4]: a face's corners (x, y, z, vertex index bits), 16 bytes each
     uint32_t F = 0, V = 0;
     std::vector<uint32_t> v_prefix;  // unique vertices referenced by faces [0, f+1)
     int32_t* runs = nullptr;         // (first face, first vertex, end vertex) per run
@@ -450,10 +450,15 @@ int qpso_mesh_upload(qpso_ctx* ctx, const float* soa, uint32_t n_faces, uint32_t
     build_mesh_tables(soa, n_faces, verts, order, m.v_prefix, runs);
     m.n_runs = (int)(runs.size() / 3);
     m.V = (uint32_t)(verts.size() / 3);
+    // corner table: (x, y, z) of corner j of face f, and its vertex index as the bits of w
     std::vector<float> corners((size_t)n_faces * 12, 0.0f);
-    for (size_t f = 0; f < n_faces; ++f)
-        for (int j = 0; j < 3; ++j)
+    for (size_t f = 0; f < n_faces; ++f) {
+        const int32_t ids[3] = {order[f].v0, order[f].v1, order[f].v2};
+        for (int j = 0; j < 3; ++j) {
             for (int c = 0; c < 3; ++c) corners[f * 12 + j * 4 + c] = soa[(size_t)c * 3 * n_faces + f + (size_t)j * n_faces];
+            std::memcpy(&corners[f * 12 + j * 4 + 3], &ids[j], sizeof(int32_t));
+        }
+    }
     hipError_t e = hipMalloc(&m.d, sizeof(float) * 9 * (size_t)n_faces);
     if (e == hipSuccess) e = hipMalloc(&m.corners, sizeof(float) * corners.size());
     if (e == hipSuccess) e = hipMalloc(&m.verts, sizeof(float) * verts.size());

@@ -48,7 +48,7 @@ struct QpsoSolveDev {
     int32_t V;       // unique vertices (row stride of verts)
     int32_t V_eff;   // unique vertices referenced by faces < F_eff (a prefix)
     const QpsoFaceRef* faces;
-    const float* corners;  // [F][3][4]: face f's corner j at corners[12 f + 4 j], (x, y, z, 0)
+    const float* corners;  // [F][3][4]: face f's corner j at corners[12 f + 4 j], (x, y, z, vertex index bits)
     int32_t n_faces; // entries in `faces` (= F)
     // runs of consecutive faces whose corners lie in a short vertex range:
     // (first face, first vertex, end vertex) per run, for meshes whose

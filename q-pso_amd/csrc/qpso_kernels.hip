@@ -462,6 +462,30 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             face_project(S.corners, e.face, pose, cam, fc);
         }
     };
+    // a face to draw: with the packed cache its corner entries carry the
+    // vertex indices (w), so the face table entry is not read
+    auto draw_face = [&](int32_t f, Face& fc) {
+        if (!WIN && pack) {
+            typedef float f4 __attribute__((ext_vector_type(4)));
+            const QG f4* __restrict__ cr = (const QG f4*)S.corners + 3 * (size_t)f;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const f4 c = cr[j];
+                // the component goes through a scalar first: this compiler
+                // lowers __builtin_bit_cast of a vector component to the
+                // bits of component 0
+                const float wf = c.w;
+                const uint32_t w = vP[__float_as_int(wf)];
+                fc.X[j] = (int32_t)(w << 16) >> 16;
+                fc.Y[j] = (int32_t)w >> 16;
+                float u, v;
+                project(pose, c.x, c.y, c.z, cam, u, v, fc.Z[j]);
+            }
+            face_bbox(fc);
+        } else {
+            load_face(f, load_ref(faces, f), fc);
+        }
+    };
     // per-pixel depth over [x0, x1) x [y0, y1): edge values at (x0, y0), per
     // pixel CX -= FDY, from a row's last pixel to the next row's first
     // CX += FDX + (w - 1) FDY
@@ -514,7 +538,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         }
         Face fc;
         Setup su;
-        load_face(fi >> 1, load_ref(faces, fi >> 1), fc);
+        draw_face(fi >> 1, fc);
         face_setup(fc, su);
         int32_t CX2 = wsub(wadd(su.CY2, wmul(y0 + dy - fc.miny, su.FDX23)), wmul(x0 + dx - fc.minx, su.FDY23));
         int32_t CX3 = wsub(wadd(su.CY3, wmul(y0 + dy - fc.miny, su.FDX31)), wmul(x0 + dx - fc.minx, su.FDY31));
@@ -655,7 +679,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         auto draw = [&](int32_t i) {
             Face fc;
             Setup su;
-            load_face(i, load_ref(faces, i), fc);
+            draw_face(i, fc);
             face_setup(fc, su);
             const int32_t x0 = max(fc.minx, tx0), x1 = min(fc.maxx, tx1);
             const int32_t y0 = max(fc.miny, b0), y1 = min(fc.maxy, b1);
@@ -791,8 +815,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         for (int q = tid; q < nq; q += BT) {
             Face fc;
             Setup su;
-            const int32_t f = queue[q];
-            load_face(f, load_ref(faces, f), fc);
+            draw_face(queue[q], fc);
             face_setup(fc, su);
             raster_exact(fc, su, max(fc.minx, dx0), min(fc.maxx, dx1), max(fc.miny, db0), min(fc.maxy, db1));
         }

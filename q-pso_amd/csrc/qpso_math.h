@@ -237,7 +237,7 @@ QD bool vertex_fixed(const float pose[7], float vx, float vy, float vz, const Ca
 }
 
 // Face f projected from the corner table (corner j of face f at
-// corners[12 f + 4 j], (x, y, z, 0): one 16-byte load per corner).
+// corners[12 f + 4 j], (x, y, z, vertex index): one 16-byte load per corner).
 QD void face_project(const float* __restrict__ corners, int f, const float pose[7], const Cam& c, Face& fc) {
     typedef float f4 __attribute__((ext_vector_type(4)));
     const f4* __restrict__ cr = reinterpret_cast<const f4*>(corners) + 3 * (size_t)f;
