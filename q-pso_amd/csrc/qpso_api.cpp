@@ -70,7 +70,6 @@ struct Mt19937 {
 int rand_stride(int iters) { return std::max(512, 12 * (iters + 1)); }
 
 struct Mesh {
-    float* d = nullptr;            // loadOBJ3 SoA [9F]
     float* verts = nullptr;        // unique vertices [3][V]
     QpsoFaceRef* faces = nullptr;  // interleaved raster order [F]
     float* corners = nullptr;      // [F][3][4]: a face's corners (x, y, z, vertex index bits), 16 bytes each
@@ -83,7 +82,6 @@ struct Mesh {
 };
 
 void free_mesh(Mesh& m) {
-    hipFree(m.d);
     hipFree(m.verts);
     hipFree(m.faces);
     hipFree(m.corners);
@@ -459,12 +457,10 @@ int qpso_mesh_upload(qpso_ctx* ctx, const float* soa, uint32_t n_faces, uint32_t
             std::memcpy(&corners[f * 12 + j * 4 + 3], &ids[j], sizeof(int32_t));
         }
     }
-    hipError_t e = hipMalloc(&m.d, sizeof(float) * 9 * (size_t)n_faces);
-    if (e == hipSuccess) e = hipMalloc(&m.corners, sizeof(float) * corners.size());
+    hipError_t e = hipMalloc(&m.corners, sizeof(float) * corners.size());
     if (e == hipSuccess) e = hipMalloc(&m.verts, sizeof(float) * verts.size());
     if (e == hipSuccess) e = hipMalloc(&m.faces, sizeof(QpsoFaceRef) * order.size());
     if (e == hipSuccess) e = hipMalloc(&m.runs, sizeof(int32_t) * std::max<size_t>(runs.size(), 1));
-    if (e == hipSuccess) e = hipMemcpy(m.d, soa, sizeof(float) * 9 * (size_t)n_faces, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(m.verts, verts.data(), sizeof(float) * verts.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(m.corners, corners.data(), sizeof(float) * corners.size(), hipMemcpyHostToDevice);
@@ -528,7 +524,6 @@ int qpso_solve_create(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl, c
 
     QpsoSolveDev& h = s->h;
     char* A = s->arena;
-    h.mesh = mit->second.d;
     h.F = (int32_t)mit->second.F;
     h.F_eff = (p->face_cap > 0 && (uint32_t)p->face_cap < mit->second.F) ? p->face_cap : (int32_t)mit->second.F;
     h.verts = mit->second.verts;

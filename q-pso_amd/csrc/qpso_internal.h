@@ -39,9 +39,8 @@ struct QpsoFaceRef {
 };
 
 struct QpsoSolveDev {
-    // mesh (metres, loadOBJ3 SoA)
-    const float* mesh;
-    int32_t F;       // faces in the SoA (row stride)
+    // mesh (metres)
+    int32_t F;       // faces
     int32_t F_eff;   // faces rendered (face cap)
     // deduplicated mesh: unique vertices [3][V] (x, y, z) and the raster order
     const float* verts;
