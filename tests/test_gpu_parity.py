@@ -180,6 +180,31 @@ def test_solve_config1_short(ctx, scene1):
     check_solve(*run_solve_pair(ctx, scene, params))
 
 
+def test_solve_config1_full_size(ctx):
+    # the benchmark's solve itself (10,336 faces, 256 particles, 100
+    # iterations): every iteration's fitness, argmin and gbest, the final
+    # swarm and the best pose, bit for bit
+    scene, params = W.config1(0)
+    mid = ctx.upload_mesh(scene.mesh_m())
+    try:
+        with ctx.solve(mid, scene.cluster(), params, trace=True) as s:
+            res = s.run()
+            tr = s.trace()
+    finally:
+        ctx.release_mesh(mid)
+    op = W.oracle_params(O, scene, params)
+    rand = O.rand_table(params.seed, params.n_particles, params.n_iters)
+    ores = O.solve(scene.mesh_m(), scene.clusters[0]["depth"], op, rand, n_threads=16)
+    assert res.status == 0
+    assert_same_f32(tr["fit_trace"], ores["fit_trace"], "fitness trace")
+    assert_same_i32(tr["argmin_trace"], ores["argmin_trace"], "argmin trace")
+    assert_same_f32(tr["gbest_trace"], ores["gbest_trace"], "gbest trace")
+    for k in ("pos", "vel", "pbest", "pbest_fit"):
+        assert_same_f32(tr[k], ores[k], k)
+    assert_same_f32(np.array(res.pose[:]), ores["best_pose"], "best pose")
+    assert res.raster_px == ores["raster_px"] and res.visit_px == ores["visit_px"]
+
+
 def test_solve_config1_face_cap(ctx, scene1):
     # the reference's 3,072-face cap on the packed vertex cache: only the
     # vertices of the first 3,072 faces are cached and drawn
