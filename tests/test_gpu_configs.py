@@ -1,6 +1,6 @@
 """GPU against the oracle on the other workloads at their sizes: the batched
-16-object frame (every solve, all 60 iterations) and the 2,048-particle swarm
-on the 100k-face mesh (its first iterations)."""
+16-object frame (every solve, all 60 iterations), the 2,048-particle swarm
+on the 100k-face mesh (its first iterations) and a frame of the served mix."""
 import numpy as np
 import pytest
 
@@ -62,3 +62,32 @@ def test_large_swarm_matches_oracle():
     for k in ("pos", "vel", "pbest", "pbest_fit"):
         assert_same_f32(tr[k], o[k], k)
     assert res.raster_px == o["raster_px"] and res.visit_px == o["visit_px"]
+
+
+def test_served_frame_matches_oracle():
+    # one frame of the served mix (4..16 clusters over 8 resident meshes of
+    # 1k..50k faces, the larger ones drawn window by window), submitted to the
+    # continuous-batching server together: each result is the oracle's
+    meshes = W.config3_meshes(0)
+    metres = [(m * np.float32(1e-3)).astype(np.float32) for m, _ in meshes]
+    scene, ids = W.config3_frame(meshes, np.random.default_rng(0), 100)
+    params = qpso.make_params(128, 10, w=0.7, c1=1.5, c2=1.5, cam=W.CAM, seed=0)
+    jobs = [j for j in range(len(ids)) if scene.clusters[j] is not None]
+    with qpso.Context(0) as ctx:
+        mids = [ctx.upload_mesh(m) for m in metres]
+        sv = qpso.Server(ctx, 8 << 30)
+        try:
+            tickets = [sv.submit(mids[ids[j]], scene.cluster(j), params) for j in jobs]
+            res = [sv.wait(t) for t in tickets]
+        finally:
+            sv.close()
+            for m in mids:
+                ctx.release_mesh(m)
+    for j, r in zip(jobs, res):
+        op = W.oracle_params(O, scene, params, j)
+        rand = O.rand_table(params.seed, params.n_particles, params.n_iters)
+        o = O.solve(metres[ids[j]], scene.clusters[j]["depth"], op, rand, n_threads=16)
+        assert r.status == 0, f"cluster {j}"
+        assert_same_f32(np.array(r.pose[:]), o["best_pose"], f"cluster {j} best pose")
+        assert_same_f32(np.float32(r.fit), o["best_fit"], f"cluster {j} best fit")
+        assert r.raster_px == o["raster_px"] and r.visit_px == o["visit_px"], f"cluster {j} work counts"
