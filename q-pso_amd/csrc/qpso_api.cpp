@@ -822,7 +822,7 @@ int qpso_autotune(qpso_ctx* ctx, int n, qpso_solve* const* solves, int iters, qp
     if (!ctx || n <= 0 || !solves || iters <= 0 || (cap > 0 && !table)) return QPSO_E_INVAL;
     for (int i = 0; i < n; ++i)
         if (!solves[i] || solves[i]->ctx != ctx) return QPSO_E_INVAL;
-    static const int kBigFaces[] = {48, 96, 160};
+    static const int kBigFaces[] = {48, 96, 128, 160};
     const int n_cfg = qpso_launch::rf_num_configs();
     std::lock_guard<std::mutex> g(ctx->mu);
     HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -864,7 +864,8 @@ int qpso_autotune(qpso_ctx* ctx, int n, qpso_solve* const* solves, int iters, qp
     };
     int best_cfg = 0, best_big = kBigFaces[0], k = 0;
     float best_ms = 0.0f;
-    for (int b = 0; b < 3 && e == hipSuccess; ++b) {
+    constexpr int kNBig = (int)(sizeof(kBigFaces) / sizeof(kBigFaces[0]));
+    for (int b = 0; b < kNBig && e == hipSuccess; ++b) {
         for (auto& d : descs) d.big_face = kBigFaces[b];
         e = hipMemcpy(d_descs, descs.data(), sizeof(QpsoSolveDev) * n, hipMemcpyHostToDevice);
         if (b == 0 && e == hipSuccess) e = run(0);  // warm-up

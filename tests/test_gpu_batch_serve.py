@@ -116,8 +116,8 @@ def test_autotune_keeps_results(ctx, frame):
     solves = [ctx.solve(mid, cl, p) for mid, cl, p in jobs]
     try:
         table = ctx.autotune(solves, iters=2)
-        assert len(table) == 15 and table[0]["ms"] <= table[-1]["ms"]
-        assert {r["big_face"] for r in table} == {48, 96, 160}
+        assert len(table) == 20 and table[0]["ms"] <= table[-1]["ms"]
+        assert {r["big_face"] for r in table} == {48, 96, 128, 160}
         res = ctx.run_batch(solves)
         assert all(same(a, b) for a, b in zip(res, plain))
         for s, b in zip(solves, plain):
