@@ -183,7 +183,7 @@ int qpso_autotune(qpso_ctx* ctx, int n, qpso_solve* const* solves, int iters,
  * of qpso_autotune candidates per threshold - 1) and big-face threshold
  * (>= 1) of solves and servers created from now on; a server keeps the
  * tuning it was created with.  Defaults: configuration 0 (or the
- * QPSO_RF_CFG environment variable), threshold 96.  No counterpart in the
+ * QPSO_RF_CFG environment variable), threshold 128.  No counterpart in the
  * reference. */
 int qpso_set_tuning(qpso_ctx* ctx, int config, int big_face);
 

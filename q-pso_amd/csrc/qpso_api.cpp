@@ -229,7 +229,7 @@ struct qpso_ctx {
         int rf_cfg = 0;
         hipGraphExec_t exec = nullptr;
     } plan;
-    int big_face = 96;  // default big-face threshold of new solves
+    int big_face = 128;  // default big-face threshold of new solves
 };
 
 struct qpso_solve {
@@ -980,7 +980,7 @@ struct qpso_server {
     uint64_t used = 0;
     int64_t max_particles = 8192;  // resident particles (slots)
     int64_t resident = 0;
-    int rf_cfg = 0, big_face = 96;  // the context's tuning when the server was created
+    int rf_cfg = 0, big_face = 128;  // the context's tuning when the server was created
     static constexpr int kLanes = 2;
     hipStream_t st[kLanes] = {};    // one stream per lane of resident solves
     std::thread th;
