@@ -59,35 +59,40 @@ def build():
     subprocess.check_call(["make", "-s", "-C", HERE])
 
 
+def load(path):
+    """A CDLL of an oracle build at `path`, with its prototypes set."""
+    L = C.CDLL(path)
+    P = C.c_void_p
+    L.or_mt19937_raw.argtypes = [C.c_uint32, P, C.c_size_t]
+    L.or_rand_table.argtypes = [C.c_uint32, P, C.c_size_t]
+    L.or_rand_stride.argtypes = [C.c_int]
+    L.or_rand_stride.restype = C.c_size_t
+    for n in ("or_sinf", "or_cosf", "or_acosf", "or_rcp"):
+        getattr(L, n).argtypes = [C.c_float]
+        getattr(L, n).restype = C.c_float
+    L.or_rint_sat.argtypes = [C.c_float]
+    L.or_rint_sat.restype = C.c_int32
+    L.or_project.argtypes = [P, P, P, P]
+    L.or_render.argtypes = [P, C.c_int, C.c_int, P, P, P, P, P]
+    L.or_fitness.argtypes = [P, P, C.c_uint32, P, P, P]
+    L.or_fitness.restype = C.c_float
+    L.or_argmin.argtypes = [P, C.c_int]
+    L.or_argmin.restype = C.c_int
+    L.or_init.argtypes = [P, P, P, P, P, P, P]
+    L.or_update_best.argtypes = [C.c_int, P, P, P, P, P, C.c_int]
+    L.or_update_particles.argtypes = [P, P, C.c_int, P, P, P, P]
+    L.or_kinematics.argtypes = [P, P]
+    L.or_solve.argtypes = [P, C.c_int, P, P, P, C.c_int, P]
+    L.or_solve.restype = C.c_int
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        P = C.c_void_p
-        L.or_mt19937_raw.argtypes = [C.c_uint32, P, C.c_size_t]
-        L.or_rand_table.argtypes = [C.c_uint32, P, C.c_size_t]
-        L.or_rand_stride.argtypes = [C.c_int]
-        L.or_rand_stride.restype = C.c_size_t
-        for n in ("or_sinf", "or_cosf", "or_acosf", "or_rcp"):
-            getattr(L, n).argtypes = [C.c_float]
-            getattr(L, n).restype = C.c_float
-        L.or_rint_sat.argtypes = [C.c_float]
-        L.or_rint_sat.restype = C.c_int32
-        L.or_project.argtypes = [P, P, P, P]
-        L.or_render.argtypes = [P, C.c_int, C.c_int, P, P, P, P, P]
-        L.or_fitness.argtypes = [P, P, C.c_uint32, P, P, P]
-        L.or_fitness.restype = C.c_float
-        L.or_argmin.argtypes = [P, C.c_int]
-        L.or_argmin.restype = C.c_int
-        L.or_init.argtypes = [P, P, P, P, P, P, P]
-        L.or_update_best.argtypes = [C.c_int, P, P, P, P, P, C.c_int]
-        L.or_update_particles.argtypes = [P, P, C.c_int, P, P, P, P]
-        L.or_kinematics.argtypes = [P, P]
-        L.or_solve.argtypes = [P, C.c_int, P, P, P, C.c_int, P]
-        L.or_solve.restype = C.c_int
-        _lib = L
+        _lib = load(LIB_PATH)
     return _lib
 
 
