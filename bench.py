@@ -2,30 +2,38 @@
 """Benchmark of the gfx950 Q-PSO pose-search hot path on one MI355X.
 
 Headline workload: one plain solve of a 10,336-face ellipsoid with 8 box
-protrusions (ROI about 200x130 px), 256 particles, 100 iterations, inputs
+protrusions (cluster rect 220x220 px), 256 particles, 100 iterations, inputs
 device-resident before timing.  One step = one whole solve (InitQPSO + 100 x
-(render+fitness, argmin/pbest/gbest/update)).  value = solves/s.
+(render+fitness, argmin/pbest/gbest/update)), one graph launch.  value =
+solves/s.
 
 Also reported on the same JSON line:
+  kernel_ms     per-kernel durations over further solves in the same process,
+                from start/stop events stamped by each kernel's own dispatch
   roofline      the render+fitness kernel against HBM: unique bytes per launch
-                (mesh, observed ROI, swarm state) / measured average launch
-                time; traffic = HBM bytes per launch from rocprof
+                (mesh, observed ROI, swarm state) / measured launch time;
+                traffic = HBM bytes per launch from rocprof (committed summary)
   roofline_lds  the same kernel against LDS bandwidth: algorithmic LDS bytes
                 per launch (4 B per bbox-fill atomicMin + 10 B per fitness
                 visit, counted exactly) / launch time
   roofline_valu the same kernel against VALU issue, the resource that binds
                 it: VALU wave instructions per launch (rocprof SQ_INSTS_VALU,
                 committed summary) / launch time
-  cpu_baseline  the CPU oracle (C, OpenMP over particles) on a bounded sample
+  cpu_baseline  the CPU oracle (C, OpenMP over particles) on the headline
+                solve; cpu_baselines: the same for the 1,280-face solve and
+                a sample of the 16-object frame
   batched       16 uneven solves of one frame: batched graph form vs plain,
                 and vs the reference's launch structure (one launch per particle)
-  served        Poisson arrivals into the continuous-batching server: p50/p99
+  served        Poisson arrivals into the continuous-batching server:
+                saturation capacity, p50/p99 over each run's steady window,
+                and the plain one-request-per-call rate of the same mix
+  stress        64 solves of 2,048 particles x 200 iterations on a 100k-face
+                mesh mixed with 64 small solves, through the server
 """
 import argparse
 import json
 import os
 import sys
-import threading
 import time
 
 REPO = os.path.dirname(os.path.abspath(__file__))
@@ -41,40 +49,84 @@ HBM_PEAK_GBS = 8000.0
 # 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 VALU instruction per SIMD every 4
 # clocks (16 lanes wide, packed FP32 off)
 VALU_PEAK_TIPS = 256 * 4 * 2.4e9 / 4 / 1e12
+PROFILE = os.path.join(REPO, "profiles", "k_render_fitness_10k_faces_220px_roi_256_particles_pmc.json")
 
 
 def pct(v, q):
     v = sorted(v)
-    return v[min(len(v) - 1, int(round(q * (len(v) - 1))))]
+    return v[min(len(v) - 1, int(round(q * (len(v) - 1))))] if v else None
 
 
 def from_profiles(key):
     """A per-launch figure of the render+fitness kernel measured by rocprofv3
     --pmc on the headline workload (committed summary), or None."""
-    path = os.path.join(REPO, "profiles", "k_render_fitness_10k_faces_256_particles_pmc.json")
-    if not os.path.exists(path):
+    if not os.path.exists(PROFILE):
         return None
     try:
-        with open(path) as f:
+        with open(PROFILE) as f:
             return float(json.load(f)[key])
     except (OSError, ValueError, KeyError):
         return None
 
 
-def cpu_baseline(scene, params, iters_sample):
-    import oracle as O
-    O.lib()
-    threads = min(16, os.cpu_count() or 1)
-    op = W.oracle_params(O, scene, params)
-    op.n_iters = iters_sample
+def host_cpus():
+    """(threads the oracle runs on, nproc, CPU model).  A one-GPU job gets a
+    share of the host: OMP_NUM_THREADS where the launcher sets it, else the
+    CPUs this process may run on."""
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS") or allowed)
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, os.cpu_count(), allowed, model
+
+
+def cpu_solve_rate(O, scene, params, i=0, iters=None, threads=1):
+    """Oracle solves/s on `iters` of the solve's iterations, scaled to whole solves."""
+    op = W.oracle_params(O, scene, params, i)
+    if iters is not None:
+        op.n_iters = iters
     rand = O.rand_table(params.seed, params.n_particles, params.n_iters)
     t = time.perf_counter()
-    O.solve(scene.mesh_m(), scene.clusters[0]["depth"], op, rand, n_threads=threads)
+    O.solve(scene.mesh_m(i), scene.clusters[i]["depth"], op, rand, n_threads=threads)
     dt = time.perf_counter() - t
-    return {"value": (iters_sample / params.n_iters) / dt, "unit": "solves/s", "cores": threads,
-            "kind": "port",
-            "sample": f"first {iters_sample} of {params.n_iters} iterations of the same solve "
-                      f"({params.n_particles} particles), {dt:.1f} s, scaled to whole solves"}
+    return (op.n_iters / params.n_iters) / dt, dt
+
+
+def cpu_baselines(scene1, params1):
+    import oracle as O
+    O.lib()
+    threads, nproc, allowed, model = host_cpus()
+    rate, dt = cpu_solve_rate(O, scene1, params1, threads=threads)
+    head = {"value": rate, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"one whole solve of the headline workload ({params1.n_particles} particles, "
+                      f"{params1.n_iters} iterations), {dt:.1f} s",
+            "nproc": nproc, "cpus_allowed": allowed, "cpu_model": model}
+    s0, p0 = W.config0(0)
+    r0s = [cpu_solve_rate(O, s0, p0, threads=threads) for _ in range(3)]
+    s2, p2 = W.config2(0)
+    idx = [i for i in range(len(p2)) if s2.clusters[i] is not None]
+    t2 = 0.0
+    for i in idx:
+        _, dt2 = cpu_solve_rate(O, s2, p2[i], i, iters=10, threads=threads)
+        t2 += dt2 * p2[i].n_iters / 10
+    extra = {
+        "config0": {"value": float(np.median([r for r, _ in r0s])), "unit": "solves/s", "cores": threads,
+                    "sample": "3 whole solves (1,280 faces, 32 particles, 30 iterations), median"},
+        "config2_frame": {"value": len(idx) / t2, "unit": "solves/s", "cores": threads,
+                          "sample": f"first 10 of 60 iterations of each of the frame's {len(idx)} solves, "
+                                    "scaled to whole solves"},
+    }
+    return head, extra
 
 
 def batched_section(ctx, reps=3):
@@ -115,13 +167,39 @@ def batched_section(ctx, reps=3):
     return {"workload": f"one frame, {n} solves, faces {min(len(o[0]) // 9 for o in scene.objects)}.."
                         f"{max(len(o[0]) // 9 for o in scene.objects)}, particles 64..512, 60 iterations",
             "solves": n, "batched_solves_per_s": n / t_b, "plain_solves_per_s": n / t_p,
-            "speedup": t_p / t_b, "reference_launch_solves_per_s": n / t_r,
-            "speedup_vs_reference_launches": t_r / t_b, "identical_results": bool(same), "autotune": tune[0]}
+            "batched_vs_plain": t_p / t_b, "reference_launch_solves_per_s": n / t_r,
+            "batched_vs_reference_launches": t_r / t_b, "identical_results": bool(same), "autotune": tune[0]}
 
 
-def served_section(ctx, lams=(200.0, 1000.0, 4000.0), seconds=1.5, seed=0, slots=8192):
-    """Poisson arrivals of (frame, cluster) requests into the continuous-batching
-    server, one run per arrival rate; latency is submit -> result."""
+def run_server(ctx, jobs, arrivals, slots, budget=32 << 30, warm_frac=0.0):
+    """Submit jobs[k] at time arrivals[k] (s from start); returns stats and
+    the p50/p99 of the requests submitted after warm_frac of the run."""
+    sv = qpso.Server(ctx, budget)
+    sv.set_slots(slots)
+    tickets = []
+    t0 = time.perf_counter()
+    for k, (mid, cl, p) in enumerate(jobs):
+        t_next = t0 + arrivals[k]
+        while True:
+            now = time.perf_counter()
+            if now >= t_next:
+                break
+            time.sleep(min(0.0005, t_next - now))
+        tickets.append(sv.submit(mid, cl, p))
+    res = [sv.wait(t) for t in tickets]
+    st = sv.stats()
+    sv.close()
+    t_end = arrivals[-1] if len(arrivals) else 0.0
+    lat = [r.latency_ms for r, a in zip(res, arrivals) if a >= warm_frac * t_end]
+    return st, res, pct(lat, 0.5), pct(lat, 0.99)
+
+
+def served_section(ctx, lams=(200.0, 1000.0, 4000.0), seconds=2.5, seed=0, slots=8192):
+    """(frame, cluster) requests into the continuous-batching server: the
+    saturation capacity (every request submitted at once), then Poisson
+    arrivals per rate; p50/p99 over requests submitted after the first 30 %
+    of each run (the steady window).  A rate above the capacity is an
+    overload: its queue grows for the whole run."""
     meshes = W.config3_meshes(seed)
     mids = [ctx.upload_mesh((m * np.float32(1e-3)).astype(np.float32)) for m, _ in meshes]
     rng = np.random.default_rng(seed)
@@ -142,36 +220,64 @@ def served_section(ctx, lams=(200.0, 1000.0, 4000.0), seconds=1.5, seed=0, slots
     finally:
         for s in sample:
             s.close()
+    # plain form: one qpso_estimate_pose call per request, one after another
+    n_plain = min(len(pool), 48)
+    t = time.perf_counter()
+    for mid, cl in pool[:n_plain]:
+        ctx.estimate_pose(mid, cl, params)
+    plain_rate = n_plain / (time.perf_counter() - t)
+    # saturation: every request at once
+    n_sat = 1500
+    jobs = [(pool[k % len(pool)][0], pool[k % len(pool)][1], params) for k in range(n_sat)]
+    st, _, _, _ = run_server(ctx, jobs, np.zeros(n_sat), slots)
+    capacity = st.solves_per_s
     runs = []
     for lam in lams:
-        sv = qpso.Server(ctx, 32 << 30)
-        sv.set_slots(slots)
         n_req = int(lam * seconds)
-        gaps = rng.exponential(1.0 / lam, n_req)
-        tickets = []
-        t0 = time.perf_counter()
-        t_next = t0
-        for k in range(n_req):
-            t_next += gaps[k]
-            while True:
-                now = time.perf_counter()
-                if now >= t_next:
-                    break
-                time.sleep(min(0.0005, t_next - now))
-            mid, cl = pool[k % len(pool)]
-            tickets.append(sv.submit(mid, cl, params))
-        for tk in tickets:
-            sv.wait(tk)
-        st = sv.stats()
-        sv.close()
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, n_req))
+        jobs = [(pool[k % len(pool)][0], pool[k % len(pool)][1], params) for k in range(n_req)]
+        st, _, p50, p99 = run_server(ctx, jobs, arrivals, slots, warm_frac=0.3)
         runs.append({"lambda_per_s": lam, "requests": n_req, "completed": int(st.completed),
-                     "solves_per_s": st.solves_per_s, "p50_ms": st.p50_ms, "p99_ms": st.p99_ms,
-                     "mean_resident_solves": st.mean_active})
+                     "solves_per_s": st.solves_per_s, "p50_ms": p50, "p99_ms": p99,
+                     "mean_resident_solves": st.mean_active, "overload": bool(lam > capacity)})
     for m in mids:
         ctx.release_mesh(m)
-    return {"workload": f"Poisson arrivals for {seconds:.1f} s per rate, 8 resident meshes 1k..50k faces, "
-                        f"128 particles, 60 iterations, 32 GiB budget, {slots} particle slots",
-            "runs": runs, "autotune": tune[0]}
+    return {"workload": f"8 resident meshes 1k..50k faces, 4..16 clusters per frame, 128 particles, 60 iterations, "
+                        f"32 GiB budget, {slots} particle slots; Poisson arrivals for {seconds:.1f} s per rate",
+            "capacity_solves_per_s": capacity, "plain_solves_per_s": plain_rate,
+            "served_vs_plain": capacity / plain_rate, "runs": runs, "autotune": tune[0]}
+
+
+def stress_section(ctx, n_stress=64, n_small=64):
+    """The stress load (64 solves of 2,048 particles x 200 iterations on the
+    100k-face mesh, mixed with 64 small 1k-face / 64-particle solves of 20..200
+    iterations) submitted at once under a 32 GiB budget; and the plain rate
+    of the same stress solve (one qpso_estimate_pose call)."""
+    big, small, jobs = W.config4_mix(0, n_stress, n_small)
+    scenes = [big, small]
+    mids = [ctx.upload_mesh(sc.mesh_m()) for sc in scenes]
+    js = [(mids[k], scenes[k].cluster(), p) for k, p in jobs]
+    slots = sum(p.n_particles for _, p in jobs)
+    st, res, p50, p99 = run_server(ctx, js, np.zeros(len(js)), slots)
+    ok = all(r.status == 0 for r in res)
+    lat_big = [r.latency_ms for (k, _), r in zip(jobs, res) if k == 0]
+    lat_small = [r.latency_ms for (k, _), r in zip(jobs, res) if k == 1]
+    t = time.perf_counter()
+    for _, p in [j for j in jobs if j[0] == 0][:2]:
+        ctx.estimate_pose(mids[0], big.cluster(), p)
+    plain_big = 2 / (time.perf_counter() - t)
+    for m in mids:
+        ctx.release_mesh(m)
+    rect = big.clusters[0]["rect"]
+    return {"workload": f"{n_stress} solves of a {big.objects[0][0].size // 9}-face mesh (ROI "
+                        f"{rect[2] - rect[0]}x{rect[3] - rect[1]}, 2048 particles, 200 iterations, sigma 4 mm, "
+                        f"15% holes) + {n_small} solves of a {small.objects[0][0].size // 9}-face mesh "
+                        f"(64 particles, 20..200 iterations), submitted at once, 32 GiB budget",
+            "solves": len(jobs), "all_ok": ok, "solves_per_s": st.solves_per_s,
+            "stress_solves_per_s": n_stress / (max(lat_big) * 1e-3) if lat_big else None,
+            "p50_ms": p50, "p99_ms": p99, "stress_p50_ms": pct(lat_big, 0.5), "stress_p99_ms": pct(lat_big, 0.99),
+            "small_p50_ms": pct(lat_small, 0.5), "small_p99_ms": pct(lat_small, 0.99),
+            "mean_resident_solves": st.mean_active, "plain_stress_solves_per_s": plain_big}
 
 
 def main():
@@ -179,11 +285,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--timed-solves", type=int, default=5, help="solves timed per kernel with HIP events")
+    ap.add_argument("--timed-solves", type=int, default=5, help="solves timed per kernel")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=10)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-stress", action="store_true")
     ap.add_argument("--lam", type=str, default="200,1000,4000", help="arrival rates of the served section")
     ap.add_argument("--slots", type=int, default=8192, help="server particle slots (particles rendered per step)")
     args = ap.parse_args()
@@ -211,8 +317,8 @@ def main():
         lat.append((time.perf_counter() - t) * 1e3)
     wall = time.perf_counter() - t0
     value = args.steps / wall
-    # per-kernel durations: HIP events between every two kernels on the
-    # launch stream, over the same solve right after the timed region
+    # per-kernel durations over further solves of the same workload: every
+    # launch stamps a start and a stop event with its own dispatch
     k1_ms = k2_ms = 0.0
     for _ in range(args.timed_solves):
         r, tm = solve.run_timed()
@@ -257,11 +363,13 @@ def main():
     solve.close()
     ctx.release_mesh(mid)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(scene, params, args.cpu_iters)
+        out["cpu_baseline"], out["cpu_baselines"] = cpu_baselines(scene, params)
     if not args.no_batched:
         out["batched"] = batched_section(ctx)
     if not args.no_served:
         out["served"] = served_section(ctx, lams=[float(x) for x in args.lam.split(",")], slots=args.slots)
+    if not args.no_stress:
+        out["stress"] = stress_section(ctx)
     ctx.close()
     print(json.dumps(out))
 

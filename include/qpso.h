@@ -39,7 +39,9 @@ enum {
     QPSO_E_BUDGET = -4,    /* request does not fit the server's HBM budget */
     QPSO_E_NOTFOUND = -5,  /* unknown mesh id / ticket */
     QPSO_E_STALE = -6,     /* a particle's stale-pixel list overflowed */
-    QPSO_E_PENDING = -7    /* ticket not finished yet */
+    QPSO_E_PENDING = -7,   /* ticket not finished yet */
+    QPSO_E_CANCELLED = -8, /* request cancelled (qpso_server_cancel, qpso_server_destroy) */
+    QPSO_E_TIMEOUT = -9    /* request not finished within the server's time-out */
 };
 
 typedef struct qpso_ctx qpso_ctx;
@@ -83,6 +85,7 @@ typedef struct {
     uint64_t visit_px;     /* fitness pixel visits over the solve (P_f) */
     uint32_t n_degenerate; /* particle evaluations scored NaN by the guard */
     int32_t status;        /* QPSO_OK or a per-solve error */
+    double latency_ms;     /* served requests: submit to result; 0 otherwise */
 } qpso_result;
 
 /* Optional per-iteration traces of a solve (all host buffers, nullable). */
@@ -141,10 +144,12 @@ int qpso_solve_destroy(qpso_solve* s);
  * this library's, so results equal qpso_solve_run's. */
 int qpso_solve_run_reference_launches(qpso_solve* s, qpso_result* out);
 
-/* qpso_solve_run with HIP events recorded on the launch stream around every
- * kernel (the reference's myTimer brackets, main.cu:1617-1733, per kernel). */
+/* qpso_solve_run with every kernel timed (the reference's myTimer brackets,
+ * main.cu:1617-1733, per kernel): each launch stamps a start and a stop event
+ * with its own dispatch, so the figures are kernel durations without event
+ * records between the kernels. */
 typedef struct {
-    double ms_solve;          /* first to last event of the solve */
+    double ms_solve;          /* start of InitQPSO to the end of the last kernel */
     double ms_render_fitness; /* sum over the I render+fitness launches */
     double ms_pso;            /* sum over the I argmin/pbest/gbest/update launches */
     double ms_init;           /* InitQPSO */
@@ -201,12 +206,18 @@ int qpso_solve_eval_poses(qpso_solve* s, int n, const float* poses,
  * while others run joins them at the next step.  Inputs are copied at submit. */
 typedef struct {
     uint64_t submitted, completed, rejected;
-    double p50_ms, p99_ms, mean_ms;  /* submit -> result latency */
+    double p50_ms, p99_ms, mean_ms;  /* submit -> result latency of completed requests */
     double solves_per_s;             /* completed / (last completion - first submit) */
     uint64_t steps;
     double mean_active;              /* mean solves resident per step */
+    uint64_t cancelled, timed_out;
 } qpso_server_stats;
 
+/* The budget covers each resident solve's device bytes (qpso_solve_bytes)
+ * plus each random table in use (4 S N bytes, S = max(512, 12 (I + 1)),
+ * shared by solves of equal seed, N and I) once while resident solves use
+ * it.  A request that could never fit is refused at submit (QPSO_E_BUDGET).
+ * Requests are admitted in arrival order. */
 int qpso_server_create(qpso_ctx* ctx, uint64_t hbm_budget_bytes, qpso_server** out);
 /* Slots: the most particles resident at once (default 8192).  A step runs
  * one iteration of every resident solve, so fewer slots mean shorter steps
@@ -218,12 +229,29 @@ int qpso_server_create(qpso_ctx* ctx, uint64_t hbm_budget_bytes, qpso_server** o
 int qpso_server_set_slots(qpso_server* sv, int64_t max_resident_particles);
 int qpso_server_submit(qpso_server* sv, uint32_t mesh_id, const qpso_cluster* cl,
                        const qpso_params* p, uint64_t* ticket);
+/* A ticket's result is read once, by poll (QPSO_E_PENDING while it runs) or
+ * wait (blocks); both return the result's status, or QPSO_E_NOTFOUND for an
+ * unknown or already read ticket. */
 int qpso_server_poll(qpso_server* sv, uint64_t ticket, qpso_result* out);
 int qpso_server_wait(qpso_server* sv, uint64_t ticket, qpso_result* out);
+/* Cancel a request: a queued one ends at once, a resident one at the next
+ * step boundary, with status QPSO_E_CANCELLED and its memory released.  A
+ * finished one keeps its result.  QPSO_E_NOTFOUND for unknown or read
+ * tickets.  No counterpart in the reference. */
+int qpso_server_cancel(qpso_server* sv, uint64_t ticket);
+/* Requests submitted from now on that have not finished `timeout_ms` after
+ * their submission end with QPSO_E_TIMEOUT (checked at step boundaries);
+ * 0 (the default) disables it.  No counterpart in the reference. */
+int qpso_server_set_timeout(qpso_server* sv, double timeout_ms);
 int qpso_server_stats_get(qpso_server* sv, qpso_server_stats* st);
+/* Ends the requests that have no result yet with QPSO_E_CANCELLED, releases
+ * threads blocked in qpso_server_wait, and frees the server.  No other call
+ * on the server may start once destroy has begun. */
 int qpso_server_destroy(qpso_server* sv);
 
-/* Device bytes one solve with these parameters holds (admission unit). */
+/* Device bytes of one solve's own arena (observed crop, swarm state, carried
+ * pixels), the admission unit; its random table is shared and counted
+ * separately (see qpso_server_create). */
 uint64_t qpso_solve_bytes(const qpso_params* p, const int32_t rect[4]);
 
 #ifdef __cplusplus

@@ -6,8 +6,9 @@
  * GraphCannySeg.cpp), which only supply the per-cluster ROI that the
  * estimate-pose block consumes (main.cu:1269-1322):
  *   rect_aabb_   -> qpso_cluster.rect (cv::Rect; br exclusive)
- *   centroid3D   -> qpso_cluster.centroid_mm (GraphCannySeg.h:2578-2605:
- *                   mean back-projection of the cluster's non-zero depths)
+ *   centroid3D   -> qpso_cluster.centroid_mm (GraphCannySeg.h:2569-2597:
+ *                   the cluster's non-zero depths back-projected through the
+ *                   float K^-1 and averaged in float)
  *   num_points   -> qpso_cluster.num_points (every cluster pixel, zero depth
  *                   included, GraphCannySeg.h:1824)
  *   clusterDepth -> the cluster-masked depth frame (GraphCannySeg.h:1805-1809)

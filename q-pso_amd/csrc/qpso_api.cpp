@@ -27,6 +27,7 @@
 
 #include "qpso_internal.h"
 #include "qpso_kernels.h"
+#include "qpso_sched.h"
 
 namespace {
 
@@ -68,6 +69,18 @@ struct Mt19937 {
 };
 
 int rand_stride(int iters) { return std::max(512, 12 * (iters + 1)); }
+// GSL replaces seed 0 by its default 4357 (gsl_rng_set): both name one table
+uint32_t rand_seed(uint32_t seed) { return seed == 0 ? 4357u : seed; }
+
+// One device random table, shared by the solves of equal (seed, N, S).
+struct RandTable {
+    float* d = nullptr;
+    size_t bytes = 0;
+    int refs = 0;           // solves using it
+    uint64_t last_use = 0;  // for eviction of unused tables
+};
+// unused tables kept for later solves, at most this many bytes
+constexpr size_t kRandIdleBytes = size_t(256) << 20;
 
 struct Mesh {
     float* verts = nullptr;        // unique vertices [3][V]
@@ -216,7 +229,8 @@ struct qpso_ctx {
     std::string err;
     std::map<uint32_t, Mesh> meshes;
     uint32_t next_mesh = 1;
-    std::map<std::tuple<uint32_t, int, int>, float*> rand_cache;
+    std::map<qpso_sched::TableKey, RandTable> rand_cache;
+    uint64_t rand_clock = 0;
     int rf_cfg = 0;
     int n_cu = 0;
     // cached batch plan (graph of a whole batched run)
@@ -242,6 +256,7 @@ struct qpso_solve {
     qpso_params params{};
     int32_t rect[4]{};
     uint32_t mesh_id = 0;
+    qpso_sched::TableKey rand_key{};  // its (shared) random table
     int rf_cfg = 0;                  // render+fitness configuration (qpso_autotune)
     hipGraphExec_t exec = nullptr;   // captured plain run: init + I x (render+fitness, PSO)
     std::vector<hipEvent_t> timed_ev;     // qpso_solve_run_timed's events, kept with the solve
@@ -273,10 +288,18 @@ int validate(const qpso_cluster* cl, const qpso_params* p) {
     return QPSO_OK;
 }
 
-int get_rand(qpso_ctx* ctx, uint32_t seed, int N, int S, float** out) {
-    auto key = std::make_tuple(seed, N, S);
+// The random table of (seed, N, S), generated and uploaded on first use; the
+// caller holds a reference until release_rand.  Caller holds ctx->mu.
+int get_rand(qpso_ctx* ctx, const qpso_sched::TableKey& key, float** out) {
     auto it = ctx->rand_cache.find(key);
-    if (it != ctx->rand_cache.end()) { *out = it->second; return QPSO_OK; }
+    if (it != ctx->rand_cache.end()) {
+        ++it->second.refs;
+        it->second.last_use = ++ctx->rand_clock;
+        *out = it->second.d;
+        return QPSO_OK;
+    }
+    const uint32_t seed = std::get<0>(key);
+    const int N = std::get<1>(key), S = std::get<2>(key);
     if ((size_t)S * N > kMaxRand) return set_err(ctx, QPSO_E_NOMEM, "random table too large");
     std::vector<float> h((size_t)S * N);
     Mt19937 mt(seed);
@@ -287,9 +310,33 @@ int get_rand(qpso_ctx* ctx, uint32_t seed, int N, int S, float** out) {
         hipFree(d);
         return set_err(ctx, QPSO_E_HIP, "random table upload failed");
     }
-    ctx->rand_cache[key] = d;
+    RandTable& t = ctx->rand_cache[key];
+    t.d = d;
+    t.bytes = h.size() * sizeof(float);
+    t.refs = 1;
+    t.last_use = ++ctx->rand_clock;
     *out = d;
     return QPSO_OK;
+}
+
+// Drop a solve's reference; unused tables beyond kRandIdleBytes are freed,
+// least recently used first.  Caller holds ctx->mu, and no queued work reads
+// an unused table (its last solve was synchronised before release).
+void release_rand(qpso_ctx* ctx, const qpso_sched::TableKey& key) {
+    auto it = ctx->rand_cache.find(key);
+    if (it == ctx->rand_cache.end() || --it->second.refs > 0) return;
+    for (;;) {
+        size_t idle = 0;
+        auto lru = ctx->rand_cache.end();
+        for (auto j = ctx->rand_cache.begin(); j != ctx->rand_cache.end(); ++j) {
+            if (j->second.refs > 0) continue;
+            idle += j->second.bytes;
+            if (lru == ctx->rand_cache.end() || j->second.last_use < lru->second.last_use) lru = j;
+        }
+        if (idle <= kRandIdleBytes || lru == ctx->rand_cache.end()) return;
+        hipFree(lru->second.d);
+        ctx->rand_cache.erase(lru);
+    }
 }
 
 void invalidate_plan(qpso_ctx* ctx) {
@@ -300,27 +347,38 @@ void invalidate_plan(qpso_ctx* ctx) {
     P = qpso_ctx::Plan{};
 }
 
-int read_result(qpso_solve* s, qpso_result* out, hipStream_t st) {
-    qpso_ctx* ctx = s->ctx;
+// A finished solve's result, read on stream `st`; touches no context state
+// (the serving thread calls it without the context lock).
+int read_result_status(qpso_solve* s, qpso_result* out, hipStream_t st) {
     float g[8];
     unsigned long long c[3];
     int32_t flags = 0;
-    HIP_TRY(ctx, hipMemcpyAsync(g, s->arena + s->L.gbest, sizeof(g), hipMemcpyDeviceToHost, st));
-    HIP_TRY(ctx, hipMemcpyAsync(c, s->arena + s->L.counters, sizeof(c), hipMemcpyDeviceToHost, st));
-    HIP_TRY(ctx, hipMemcpyAsync(&flags, s->arena + s->L.flags, sizeof(flags), hipMemcpyDeviceToHost, st));
-    HIP_TRY(ctx, hipStreamSynchronize(st));
+    if (hipMemcpyAsync(g, s->arena + s->L.gbest, sizeof(g), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(c, s->arena + s->L.counters, sizeof(c), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(&flags, s->arena + s->L.flags, sizeof(flags), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return QPSO_E_HIP;
+    const int status = (flags & QPSO_FLAG_LAUNCH) ? QPSO_E_HIP
+                       : (flags & QPSO_FLAG_STALE_OVERFLOW) ? QPSO_E_STALE : QPSO_OK;
     if (out) {
         std::memcpy(out->pose, g, sizeof(float) * 7);
         out->fit = g[7];
         out->raster_px = c[0];
         out->visit_px = c[1];
         out->n_degenerate = (uint32_t)c[2];
-        out->status = (flags & QPSO_FLAG_LAUNCH) ? QPSO_E_HIP
-                      : (flags & QPSO_FLAG_STALE_OVERFLOW) ? QPSO_E_STALE : QPSO_OK;
+        out->status = status;
+        out->latency_ms = 0.0;
     }
-    if (flags & QPSO_FLAG_LAUNCH) return set_err(ctx, QPSO_E_HIP, "render kernel launched without mesh windows");
-    return (flags & QPSO_FLAG_STALE_OVERFLOW) ? set_err(ctx, QPSO_E_STALE, "stale-pixel list overflow")
-                                               : QPSO_OK;
+    return status;
+}
+
+int read_result(qpso_solve* s, qpso_result* out, hipStream_t st) {
+    const int rc = read_result_status(s, out, st);
+    switch (rc) {
+        case QPSO_OK: return QPSO_OK;
+        case QPSO_E_STALE: return set_err(s->ctx, rc, "stale-pixel list overflow");
+        default: return set_err(s->ctx, rc, "result readback failed, or a render kernel launched without mesh windows");
+    }
 }
 
 bool needs_windows(const qpso_solve* s, int cfg) { return qpso_launch::rf_needs_windows(cfg, s->h.V_eff); }
@@ -381,6 +439,8 @@ const char* qpso_status_str(int status) {
         case QPSO_E_NOTFOUND: return "unknown id";
         case QPSO_E_STALE: return "stale-pixel list overflow";
         case QPSO_E_PENDING: return "pending";
+        case QPSO_E_CANCELLED: return "cancelled";
+        case QPSO_E_TIMEOUT: return "timed out";
         default: return "unknown status";
     }
 }
@@ -428,7 +488,7 @@ int qpso_destroy(qpso_ctx* ctx) {
     hipStreamSynchronize(ctx->stream);
     invalidate_plan(ctx);
     for (auto& kv : ctx->meshes) free_mesh(kv.second);
-    for (auto& kv : ctx->rand_cache) hipFree(kv.second);
+    for (auto& kv : ctx->rand_cache) hipFree(kv.second.d);
     hipStreamDestroy(ctx->stream);
     delete ctx;
     return QPSO_OK;
@@ -520,7 +580,8 @@ int qpso_solve_create(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl, c
     if (hipMalloc(&s->arena, s->L.total) != hipSuccess)
         return set_err(ctx, QPSO_E_NOMEM, "solve arena allocation failed");
     float* rand = nullptr;
-    if (int r = get_rand(ctx, p->seed, N, S, &rand)) { hipFree(s->arena); return r; }
+    s->rand_key = qpso_sched::TableKey(rand_seed(p->seed), N, S);
+    if (int r = get_rand(ctx, s->rand_key, &rand)) { hipFree(s->arena); return r; }
 
     QpsoSolveDev& h = s->h;
     char* A = s->arena;
@@ -590,7 +651,11 @@ int qpso_solve_create(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl, c
         hipMemcpyAsync(s->d_desc, &s->h, sizeof(QpsoSolveDev), hipMemcpyHostToDevice, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
         rc = set_err(ctx, QPSO_E_HIP, "solve upload failed");
-    if (rc) { hipFree(s->arena); return rc; }
+    if (rc) {
+        hipFree(s->arena);
+        release_rand(ctx, s->rand_key);
+        return rc;
+    }
     ++mit->second.refs;
     *out = s.release();
     return QPSO_OK;
@@ -607,6 +672,7 @@ int qpso_solve_destroy(qpso_solve* s) {
     drop_solve_graph(s);
     for (auto& e : s->timed_ev) hipEventDestroy(e);
     hipFree(s->arena);
+    release_rand(ctx, s->rand_key);
     auto mit = ctx->meshes.find(s->mesh_id);
     if (mit != ctx->meshes.end() && --mit->second.refs == 0 && mit->second.released) {
         free_mesh(mit->second);
@@ -681,40 +747,35 @@ int qpso_solve_run_timed(qpso_solve* s, qpso_result* out, qpso_timing* tm) {
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     const int I = s->h.I;
-    // an event between every two kernels, launched on the stream (event
-    // records captured into a graph do not time on this runtime)
-    if (s->timed_ev.size() != 2 + 2 * (size_t)I) {
-        s->timed_ev.resize(2 + 2 * (size_t)I);
+    // a start and a stop event per kernel, stamped by the kernel's dispatch
+    // itself (no event records between the kernels)
+    const size_t n_ev = 2 * (1 + 2 * (size_t)I);
+    if (s->timed_ev.size() != n_ev) {
+        for (auto& e : s->timed_ev) hipEventDestroy(e);
+        s->timed_ev.assign(n_ev, nullptr);
         for (auto& e : s->timed_ev) HIP_TRY(ctx, hipEventCreate(&e));
     }
     const std::vector<hipEvent_t>& ev = s->timed_ev;
-    auto enqueue = [&]() {
-        hipError_t e = hipEventRecord(ev[0], st);
-        if (e == hipSuccess) e = qpso_launch::init(s->d_desc, 1, st);
-        if (e == hipSuccess) e = hipEventRecord(ev[1], st);
-        for (int it = 0; it < I && e == hipSuccess; ++it) {
-            e = qpso_launch::render_fitness(s->rf_cfg, needs_windows(s, s->rf_cfg), s->d_desc, nullptr, s->h.N, st);
-            if (e == hipSuccess) e = hipEventRecord(ev[2 + 2 * it], st);
-            if (e == hipSuccess) e = qpso_launch::pso_step(s->d_desc, 1, st);
-            if (e == hipSuccess) e = hipEventRecord(ev[3 + 2 * it], st);
-        }
-        return e;
-    };
-    hipError_t e = enqueue();
+    const bool win = needs_windows(s, s->rf_cfg);
+    hipError_t e = qpso_launch::init(s->d_desc, 1, st, ev[0], ev[1]);
+    for (int it = 0; it < I && e == hipSuccess; ++it) {
+        e = qpso_launch::render_fitness(s->rf_cfg, win, s->d_desc, nullptr, s->h.N, st, ev[2 + 4 * it],
+                                        ev[3 + 4 * it]);
+        if (e == hipSuccess) e = qpso_launch::pso_step(s->d_desc, 1, st, ev[4 + 4 * it], ev[5 + 4 * it]);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return set_err(ctx, QPSO_E_HIP, std::string("timed run: ") + hipGetErrorString(e));
-    float ms = 0;
+    auto span = [&](size_t a, size_t b) {
+        float ms = 0.0f;
+        return hipEventElapsedTime(&ms, ev[a], ev[b]) == hipSuccess ? (double)ms : 0.0;
+    };
     std::memset(tm, 0, sizeof(*tm));
-    hipEventElapsedTime(&ms, ev[0], ev[1]);
-    tm->ms_init = ms;
+    tm->ms_init = span(0, 1);
     for (int it = 0; it < I; ++it) {
-        hipEventElapsedTime(&ms, ev[1 + 2 * it], ev[2 + 2 * it]);
-        tm->ms_render_fitness += ms;
-        hipEventElapsedTime(&ms, ev[2 + 2 * it], ev[3 + 2 * it]);
-        tm->ms_pso += ms;
+        tm->ms_render_fitness += span(2 + 4 * it, 3 + 4 * it);
+        tm->ms_pso += span(4 + 4 * it, 5 + 4 * it);
     }
-    hipEventElapsedTime(&ms, ev[0], ev[1 + 2 * I]);
-    tm->ms_solve = ms;
+    tm->ms_solve = span(0, n_ev - 1);
     tm->launches = I;
     return read_result(s, out, st);
 }
@@ -974,64 +1035,31 @@ int qpso_solve_run_batch(qpso_ctx* ctx, int n, qpso_solve* const* solves, qpso_r
 }  // extern "C"
 
 // ---- serving: continuous batching under an HBM budget -----------------------
-struct qpso_server {
-    qpso_ctx* ctx = nullptr;
-    uint64_t budget = 0;
-    uint64_t used = 0;
-    int64_t max_particles = 8192;  // resident particles (slots)
-    int64_t resident = 0;
-    int rf_cfg = 0, big_face = 128;  // the context's tuning when the server was created
-    static constexpr int kLanes = 2;
-    hipStream_t st[kLanes] = {};    // one stream per lane of resident solves
-    std::thread th;
-    std::mutex mu;
-    std::condition_variable cv, done_cv;
-    bool stop = false;
-    struct Req {
-        uint64_t ticket;
-        uint32_t mesh_id;
-        qpso_params p;
-        qpso_cluster cl;
-        std::vector<uint16_t> depth;
-        uint64_t bytes;
-        std::chrono::steady_clock::time_point t_submit;
-    };
-    std::deque<Req> pending;
-    struct Active {
-        Req req;
-        qpso_solve* s;
-        int iter;  // iterations enqueued so far (every step advances each resident solve by one)
-    };
-    std::unordered_map<uint64_t, qpso_result> results;
-    std::vector<double> latencies_ms;
-    uint64_t next_ticket = 1, submitted = 0, completed = 0, rejected = 0, steps = 0;
-    double active_sum = 0;
-    std::chrono::steady_clock::time_point t_first, t_last;
-    bool have_first = false;
-};
-
 namespace {
 
-// One device step is one PSO iteration of every resident solve: a fused
-// render+fitness launch over all their particles and one PSO launch.  The
-// resident solves are split into two lanes, each with its own stream, work
-// tables and steps, so the GPU overlaps the tail of one lane's step (its last,
-// shortest workgroups) with the other lane's next step; a new solve joins the
-// lane with fewer particles.  Within a lane the largest meshes' workgroups
-// start first.  The host knows each solve's iteration count, so steps are
-// enqueued without a round trip; at most kInFlight steps per lane are queued
-// ahead of the device, so a request that arrives joins within about two
-// steps.  A lane's stream is synchronised only when one of its solves
-// completes and its result is read.
-void server_loop(qpso_server* sv) {
-    constexpr int kInFlight = 2;
-    constexpr int kLanes = qpso_server::kLanes;
-    qpso_ctx* ctx = sv->ctx;
-    hipSetDevice(ctx->device);
+// The serving loop's device: the scheduler (qpso_sched.h) decides what is
+// resident; this enqueues the work.  One device step of a lane is one PSO
+// iteration of each of its solves: a fused render+fitness launch over all
+// their particles and one PSO launch.  The two lanes have their own streams
+// and work tables, so the GPU overlaps the tail of one lane's step (its last,
+// shortest workgroups) with the other lane's next step.  Within a lane the
+// largest meshes' workgroups start first.  The host counts iterations itself,
+// so steps are enqueued without a round trip; at most kInFlight steps per
+// lane are queued ahead of the device, so a request that arrives joins within
+// about two steps.  A lane's stream is synchronised only when one of its
+// solves leaves.
+struct HipDev {
+    using Solve = qpso_solve;
+    struct Job {
+        uint32_t mesh_id = 0;
+        qpso_params p{};
+        qpso_cluster cl{};
+        std::vector<uint16_t> depth;  // the caller's frame, copied at submit
+    };
+    static constexpr int kInFlight = 2;
+    static constexpr int kLanes = 2;
     struct Lane {
         hipStream_t st = nullptr;
-        std::vector<qpso_server::Active> active;
-        int64_t particles = 0;
         QpsoSolveDev* d_descs = nullptr;
         QpsoItem* d_items = nullptr;
         size_t cap_descs = 0, cap_items = 0;
@@ -1039,45 +1067,60 @@ void server_loop(qpso_server* sv) {
         std::vector<QpsoItem> items;
         hipEvent_t ev[kInFlight] = {};
         uint64_t launched = 0;
-        bool changed = true;
     };
+    qpso_ctx* ctx;
+    int rf_cfg, big_face;  // the context's tuning when the server was created
     Lane lanes[kLanes];
-    for (int k = 0; k < kLanes; ++k) {
-        lanes[k].st = sv->st[k];
-        for (auto& e : lanes[k].ev) hipEventCreateWithFlags(&e, hipEventDisableTiming);
-    }
-    auto any_active = [&]() {
-        for (auto& l : lanes)
-            if (!l.active.empty()) return true;
-        return false;
-    };
-    // a device allocation or launch that fails ends every request of the lane
-    // with that status instead of leaving its waiter blocked
-    auto fail_lane = [&](Lane& l, int code) {
-        hipStreamSynchronize(l.st);
-        for (auto& a : l.active) {
-            qpso_solve_destroy(a.s);
-            std::lock_guard<std::mutex> lk(sv->mu);
-            qpso_result res{};
-            res.status = code;
-            sv->results[a.req.ticket] = res;
-            sv->used -= a.req.bytes;
-            sv->resident -= a.req.p.n_particles;
-            ++sv->rejected;
-            sv->done_cv.notify_all();
+
+    explicit HipDev(qpso_ctx* c) : ctx(c), rf_cfg(c->rf_cfg), big_face(c->big_face) {}
+
+    int open() {
+        if (hipSetDevice(ctx->device) != hipSuccess) return QPSO_E_HIP;
+        for (auto& l : lanes) {
+            if (hipStreamCreateWithFlags(&l.st, hipStreamNonBlocking) != hipSuccess) return QPSO_E_HIP;
+            for (auto& e : l.ev)
+                if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return QPSO_E_HIP;
         }
-        l.active.clear();
-        l.particles = 0;
-        l.changed = true;
-    };
-    // the work tables of a lane: its solves' descriptors and one item per
+        return QPSO_OK;
+    }
+    void close() {
+        hipSetDevice(ctx->device);
+        for (auto& l : lanes) {
+            if (l.st) hipStreamSynchronize(l.st);
+            for (auto& e : l.ev)
+                if (e) hipEventDestroy(e);
+            if (l.d_descs) hipFree(l.d_descs);
+            if (l.d_items) hipFree(l.d_items);
+            if (l.st) hipStreamDestroy(l.st);
+            l = Lane{};
+        }
+    }
+    void on_thread_start() { hipSetDevice(ctx->device); }
+    void on_thread_end() {}
+
+    double now() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    }
+
+    int create(Job& j, qpso_solve*& s) {
+        j.cl.depth = j.depth.data();
+        const int rc = qpso_solve_create(ctx, j.mesh_id, &j.cl, &j.p, 0, &s);
+        if (rc) return rc;
+        s->rf_cfg = rf_cfg;  // the server's tuning, not the context's current one
+        s->h.big_face = big_face;
+        return QPSO_OK;
+    }
+    int init(qpso_solve* s, int k) {
+        return qpso_launch::init(s->d_desc, 1, lanes[k].st) == hipSuccess ? QPSO_OK : QPSO_E_HIP;
+    }
+    // the lane's work tables: its solves' descriptors and one item per
     // particle, the largest meshes' particles first
-    auto rebuild = [&](Lane& l) {
+    int rebuild(Lane& l, const std::vector<qpso_solve*>& ss) {
         l.descs.clear();
         l.items.clear();
-        std::vector<int> order(l.active.size());
-        for (size_t i = 0; i < l.active.size(); ++i) {
-            l.descs.push_back(l.active[i].s->h);
+        std::vector<int> order(ss.size());
+        for (size_t i = 0; i < ss.size(); ++i) {
+            l.descs.push_back(ss[i]->h);
             order[i] = (int)i;
         }
         std::stable_sort(order.begin(), order.end(),
@@ -1085,7 +1128,8 @@ void server_loop(qpso_server* sv) {
         for (int i : order)
             for (int q = 0; q < l.descs[i].N; ++q) l.items.push_back(QpsoItem{(int32_t)i, q});
         if (l.descs.size() > l.cap_descs || l.items.size() > l.cap_items) {
-            hipStreamSynchronize(l.st);  // the old tables may still be read by queued steps
+            // the old tables may still be read by queued steps
+            if (hipStreamSynchronize(l.st) != hipSuccess) return QPSO_E_HIP;
             if (l.descs.size() > l.cap_descs) {
                 if (l.d_descs) hipFree(l.d_descs);
                 l.d_descs = nullptr;
@@ -1107,237 +1151,109 @@ void server_loop(qpso_server* sv) {
             hipMemcpyAsync(l.d_items, l.items.data(), sizeof(QpsoItem) * l.items.size(), hipMemcpyHostToDevice,
                            l.st) != hipSuccess)
             return QPSO_E_HIP;
-        l.changed = false;
         return QPSO_OK;
-    };
-    for (;;) {
-        std::vector<qpso_server::Req> admit;
-        {
-            std::unique_lock<std::mutex> lk(sv->mu);
-            sv->cv.wait(lk, [&] { return sv->stop || !sv->pending.empty() || any_active(); });
-            if (sv->stop && !any_active()) break;
-            // admit in arrival order while the budget and the slots allow (a
-            // solve larger than the slots runs when nothing else is resident)
-            while (!sv->pending.empty()) {
-                const auto& r = sv->pending.front();
-                const bool fits = sv->used + r.bytes <= sv->budget &&
-                                  (sv->resident + r.p.n_particles <= sv->max_particles ||
-                                   (sv->resident == 0 && admit.empty()));
-                if (!fits) break;
-                sv->used += r.bytes;
-                sv->resident += r.p.n_particles;
-                admit.push_back(std::move(sv->pending.front()));
-                sv->pending.pop_front();
-            }
-        }
-        for (auto& r : admit) {
-            r.cl.depth = r.depth.data();
-            qpso_solve* s = nullptr;
-            const int rc = qpso_solve_create(ctx, r.mesh_id, &r.cl, &r.p, 0, &s);
-            if (rc) {
-                std::lock_guard<std::mutex> lk(sv->mu);
-                qpso_result res{};
-                res.status = rc;
-                sv->results[r.ticket] = res;
-                sv->used -= r.bytes;
-                sv->resident -= r.p.n_particles;
-                ++sv->rejected;
-                sv->done_cv.notify_all();
-                continue;
-            }
-            s->rf_cfg = sv->rf_cfg;  // the server's tuning, not the context's current one
-            s->h.big_face = sv->big_face;
-            Lane& l = lanes[0].particles <= lanes[1].particles ? lanes[0] : lanes[1];
-            // InitQPSO on the lane's stream, ahead of the solve's first step
-            qpso_launch::init(s->d_desc, 1, l.st);
-            l.particles += s->h.N;
-            l.active.push_back({std::move(r), s, 0});
-            l.changed = true;
-        }
-        if (!any_active()) continue;
-
-        // A step with no more workgroups than CUs is latency-bound: it takes
-        // the widest configuration (one 1024-thread workgroup per CU); fuller
-        // steps take the server's tuned one.  Results do not depend on it.
-        int64_t total = 0;
-        size_t n_active = 0;
-        for (auto& l : lanes) {
-            total += l.particles;
-            n_active += l.active.size();
-        }
-        const int step_cfg = total <= (int64_t)std::max(ctx->n_cu, 1) ? 0 : sv->rf_cfg;
-        ++sv->steps;
-        sv->active_sum += (double)n_active;
-        for (auto& l : lanes) {
-            if (l.active.empty()) continue;
-            if (l.changed) {
-                if (const int rc = rebuild(l)) {
-                    fail_lane(l, rc);
-                    continue;
-                }
-            }
-            bool win = false;
-            for (auto& a : l.active) win |= needs_windows(a.s, step_cfg);
-            if (l.launched >= (uint64_t)kInFlight) hipEventSynchronize(l.ev[l.launched % kInFlight]);
-            if (qpso_launch::render_fitness(step_cfg, win, l.d_descs, l.d_items, (int)l.items.size(), l.st) !=
-                    hipSuccess ||
-                qpso_launch::pso_step(l.d_descs, (int)l.descs.size(), l.st) != hipSuccess) {
-                fail_lane(l, QPSO_E_HIP);
-                continue;
-            }
-            hipEventRecord(l.ev[l.launched % kInFlight], l.st);
-            ++l.launched;
-        }
-        for (auto& l : lanes) {
-            bool any_done = false;
-            for (auto& a : l.active) any_done |= ++a.iter >= a.s->h.I;
-            if (!any_done) continue;
-            hipStreamSynchronize(l.st);
-            std::vector<qpso_server::Active> keep;
-            for (auto& a : l.active) {
-                if (a.iter < a.s->h.I) {
-                    keep.push_back(std::move(a));
-                    continue;
-                }
-                qpso_result res{};
-                const int rc = read_result(a.s, &res, l.st);
-                if (rc && res.status == QPSO_OK) res.status = rc;
-                const auto now = std::chrono::steady_clock::now();
-                l.particles -= a.s->h.N;
-                qpso_solve_destroy(a.s);
-                std::lock_guard<std::mutex> lk(sv->mu);
-                sv->results[a.req.ticket] = res;
-                sv->latencies_ms.push_back(std::chrono::duration<double, std::milli>(now - a.req.t_submit).count());
-                sv->used -= a.req.bytes;
-                sv->resident -= a.req.p.n_particles;
-                ++sv->completed;
-                sv->t_last = now;
-                sv->done_cv.notify_all();
-            }
-            l.active.swap(keep);
-            l.changed = true;
-        }
     }
-    for (auto& l : lanes) {
-        hipStreamSynchronize(l.st);
-        for (auto& e : l.ev) hipEventDestroy(e);
-        if (l.d_descs) hipFree(l.d_descs);
-        if (l.d_items) hipFree(l.d_items);
+    // A step with no more workgroups than CUs is latency-bound: it takes the
+    // widest configuration (one 1024-thread workgroup per CU); fuller steps
+    // take the server's tuned one.  Results do not depend on it.
+    int step(int k, const std::vector<qpso_solve*>& ss, bool changed, int64_t total) {
+        Lane& l = lanes[k];
+        if (changed)
+            if (const int rc = rebuild(l, ss)) return rc;
+        const int cfg = total <= (int64_t)std::max(ctx->n_cu, 1) ? 0 : rf_cfg;
+        bool win = false;
+        for (auto* s : ss) win |= needs_windows(s, cfg);
+        if (l.launched >= (uint64_t)kInFlight && hipEventSynchronize(l.ev[l.launched % kInFlight]) != hipSuccess)
+            return QPSO_E_HIP;
+        if (qpso_launch::render_fitness(cfg, win, l.d_descs, l.d_items, (int)l.items.size(), l.st) != hipSuccess ||
+            qpso_launch::pso_step(l.d_descs, (int)l.descs.size(), l.st) != hipSuccess ||
+            hipEventRecord(l.ev[l.launched % kInFlight], l.st) != hipSuccess)
+            return QPSO_E_HIP;
+        ++l.launched;
+        return QPSO_OK;
     }
-}
+    int sync(int k) { return hipStreamSynchronize(lanes[k].st) == hipSuccess ? QPSO_OK : QPSO_E_HIP; }
+    int read(int k, qpso_solve* s, qpso_result* out) { return read_result_status(s, out, lanes[k].st); }
+    void destroy(qpso_solve* s) { qpso_solve_destroy(s); }
+};
 
 }  // namespace
+
+struct qpso_server {
+    HipDev dev;  // declared first: outlives the scheduler that drives it
+    qpso_sched::Server<HipDev> core;
+    qpso_server(qpso_ctx* ctx, uint64_t budget) : dev(ctx), core(dev, budget) {}
+};
 
 extern "C" {
 
 int qpso_server_create(qpso_ctx* ctx, uint64_t hbm_budget_bytes, qpso_server** out) {
     if (!ctx || !out || hbm_budget_bytes == 0) return QPSO_E_INVAL;
-    auto* sv = new qpso_server();
-    sv->ctx = ctx;
-    sv->budget = hbm_budget_bytes;
-    sv->rf_cfg = ctx->rf_cfg;
-    sv->big_face = ctx->big_face;
-    hipSetDevice(ctx->device);
-    for (auto& x : sv->st) {
-        if (hipStreamCreateWithFlags(&x, hipStreamNonBlocking) != hipSuccess) {
-            for (auto& y : sv->st)
-                if (y) hipStreamDestroy(y);
-            delete sv;
-            return QPSO_E_HIP;
-        }
+    *out = nullptr;
+    auto* sv = new qpso_server(ctx, hbm_budget_bytes);
+    if (sv->dev.open() != QPSO_OK) {
+        sv->dev.close();
+        delete sv;
+        return QPSO_E_HIP;
     }
-    sv->th = std::thread(server_loop, sv);
+    sv->core.start();
     *out = sv;
     return QPSO_OK;
 }
 
 int qpso_server_set_slots(qpso_server* sv, int64_t max_resident_particles) {
-    if (!sv || max_resident_particles <= 0) return QPSO_E_INVAL;
-    std::lock_guard<std::mutex> lk(sv->mu);
-    sv->max_particles = max_resident_particles;
-    sv->cv.notify_all();
-    return QPSO_OK;
+    if (!sv) return QPSO_E_INVAL;
+    return sv->core.set_slots(max_resident_particles);
+}
+
+int qpso_server_set_timeout(qpso_server* sv, double timeout_ms) {
+    if (!sv) return QPSO_E_INVAL;
+    return sv->core.set_timeout(timeout_ms * 1e-3);
 }
 
 int qpso_server_submit(qpso_server* sv, uint32_t mesh_id, const qpso_cluster* cl, const qpso_params* p,
                        uint64_t* ticket) {
     if (!sv || !ticket) return QPSO_E_INVAL;
     if (int v = validate(cl, p)) return v;
-    qpso_server::Req r;
-    r.mesh_id = mesh_id;
-    r.p = *p;
-    r.cl = *cl;
-    r.depth.assign(cl->depth, cl->depth + kFrame);
+    qpso_sched::Server<HipDev>::Request r;
+    r.job.mesh_id = mesh_id;
+    r.job.p = *p;
+    r.job.cl = *cl;
+    r.job.depth.assign(cl->depth, cl->depth + kFrame);
     r.bytes = qpso_solve_bytes(p, cl->rect);
-    r.t_submit = std::chrono::steady_clock::now();
-    std::lock_guard<std::mutex> lk(sv->mu);
-    if (r.bytes > sv->budget) { ++sv->rejected; return QPSO_E_BUDGET; }
-    r.ticket = sv->next_ticket++;
-    *ticket = r.ticket;
-    if (!sv->have_first) { sv->t_first = r.t_submit; sv->have_first = true; }
-    ++sv->submitted;
-    sv->pending.push_back(std::move(r));
-    sv->cv.notify_all();
-    return QPSO_OK;
+    const int S = rand_stride(p->n_iters);
+    r.table = qpso_sched::TableKey(rand_seed(p->seed), p->n_particles, S);
+    r.table_bytes = sizeof(float) * (uint64_t)S * (uint64_t)p->n_particles;
+    r.particles = p->n_particles;
+    r.iters = p->n_iters;
+    return sv->core.submit(std::move(r), ticket);
 }
 
 int qpso_server_poll(qpso_server* sv, uint64_t ticket, qpso_result* out) {
     if (!sv) return QPSO_E_INVAL;
-    std::lock_guard<std::mutex> lk(sv->mu);
-    auto it = sv->results.find(ticket);
-    if (it == sv->results.end()) return ticket < sv->next_ticket ? QPSO_E_PENDING : QPSO_E_NOTFOUND;
-    if (out) *out = it->second;
-    const int st = it->second.status;
-    sv->results.erase(it);
-    return st;
+    return sv->core.poll(ticket, out);
 }
 
 int qpso_server_wait(qpso_server* sv, uint64_t ticket, qpso_result* out) {
     if (!sv) return QPSO_E_INVAL;
-    std::unique_lock<std::mutex> lk(sv->mu);
-    if (ticket == 0 || ticket >= sv->next_ticket) return QPSO_E_NOTFOUND;
-    sv->done_cv.wait(lk, [&] { return sv->results.count(ticket) != 0; });
-    auto it = sv->results.find(ticket);
-    if (out) *out = it->second;
-    const int st = it->second.status;
-    sv->results.erase(it);
-    return st;
+    return sv->core.wait(ticket, out);
+}
+
+int qpso_server_cancel(qpso_server* sv, uint64_t ticket) {
+    if (!sv) return QPSO_E_INVAL;
+    return sv->core.cancel(ticket);
 }
 
 int qpso_server_stats_get(qpso_server* sv, qpso_server_stats* st) {
     if (!sv || !st) return QPSO_E_INVAL;
-    std::lock_guard<std::mutex> lk(sv->mu);
-    std::memset(st, 0, sizeof(*st));
-    st->submitted = sv->submitted;
-    st->completed = sv->completed;
-    st->rejected = sv->rejected;
-    st->steps = sv->steps;
-    st->mean_active = sv->steps ? sv->active_sum / (double)sv->steps : 0.0;
-    if (!sv->latencies_ms.empty()) {
-        std::vector<double> v = sv->latencies_ms;
-        std::sort(v.begin(), v.end());
-        auto q = [&](double f) { return v[std::min(v.size() - 1, (size_t)(f * (double)(v.size() - 1) + 0.5))]; };
-        st->p50_ms = q(0.50);
-        st->p99_ms = q(0.99);
-        double s = 0;
-        for (double x : v) s += x;
-        st->mean_ms = s / (double)v.size();
-        const double span = std::chrono::duration<double>(sv->t_last - sv->t_first).count();
-        st->solves_per_s = span > 0 ? (double)sv->completed / span : 0.0;
-    }
+    sv->core.stats(st);
     return QPSO_OK;
 }
 
 int qpso_server_destroy(qpso_server* sv) {
     if (!sv) return QPSO_E_INVAL;
-    {
-        std::lock_guard<std::mutex> lk(sv->mu);
-        sv->stop = true;
-        sv->pending.clear();
-    }
-    sv->cv.notify_all();
-    if (sv->th.joinable()) sv->th.join();
-    for (auto& x : sv->st) hipStreamDestroy(x);
+    sv->core.shutdown();
+    sv->dev.close();
     delete sv;
     return QPSO_OK;
 }

@@ -11,6 +11,8 @@
 #include <string>
 #include <vector>
 
+#include "qpso_roi.h"
+
 namespace {
 constexpr int kRows = QPSO_ROWS;
 constexpr int kCols = QPSO_COLS;
@@ -190,29 +192,21 @@ int qpso_clusters_from_labels(const uint16_t* labels, const uint16_t* depth, int
     if (!labels || !depth || !cam || !rect || !centroid_mm || !num_points || k < 0) return QPSO_E_INVAL;
     const uint16_t want = (uint16_t)(k + 1);
     int minx = kCols, miny = kRows, maxx = -1, maxy = -1;
-    uint32_t n = 0, nz = 0;
-    double sx = 0, sy = 0, sz = 0;
+    uint32_t n = 0;
+    qpso_roi::Centroid cen(qpso_roi::kinv(cam->fx, cam->fy, cam->cx, cam->cy));
     for (int y = 0; y < kRows; ++y)
         for (int x = 0; x < kCols; ++x) {
             const int i = y * kCols + x;
             const bool in = labels[i] == want;
             if (depth_out) depth_out[i] = in ? depth[i] : 0;
             if (!in) continue;
-            ++n;
+            ++n;  // every cluster pixel, holes included (GraphCannySeg.h:1824)
             minx = std::min(minx, x); maxx = std::max(maxx, x);
             miny = std::min(miny, y); maxy = std::max(maxy, y);
-            if (depth[i] == 0) continue;
-            const double z = depth[i];
-            sx += (x - cam->cx) * z / cam->fx;
-            sy += (y - cam->cy) * z / cam->fy;
-            sz += z;
-            ++nz;
+            cen.add(x, y, depth[i]);
         }
-    if (n == 0 || nz == 0) return QPSO_E_NOTFOUND;
-    rect[0] = minx; rect[1] = miny; rect[2] = maxx + 1; rect[3] = maxy + 1;
-    centroid_mm[0] = (float)(sx / nz);
-    centroid_mm[1] = (float)(sy / nz);
-    centroid_mm[2] = (float)(sz / nz);
+    if (n == 0 || !cen.result(centroid_mm)) return QPSO_E_NOTFOUND;
+    rect[0] = minx; rect[1] = miny; rect[2] = maxx + 1; rect[3] = maxy + 1;  // cv::boundingRect, br exclusive
     *num_points = n;
     return QPSO_OK;
 }

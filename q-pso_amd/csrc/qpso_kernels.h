@@ -13,8 +13,12 @@ int rf_num_configs();
 int rf_config_threads(int cfg);
 int rf_config_tile(int cfg);
 
+// Every launcher takes optional events that the launch stamps with the
+// kernel's own start and end (hipExtLaunchKernel), for per-kernel timing.
+
 // InitQPSO for every solve in d_solves[0..n).
-hipError_t init(const QpsoSolveDev* d_solves, int n_solves, hipStream_t st);
+hipError_t init(const QpsoSolveDev* d_solves, int n_solves, hipStream_t st, hipEvent_t ev0 = nullptr,
+                hipEvent_t ev1 = nullptr);
 // Whether a mesh of v_eff vertices needs the window-by-window instantiation.
 bool rf_needs_windows(int cfg, int v_eff);
 // One fused render+fitness pass; d_items[i] = (solve index, particle index),
@@ -22,9 +26,10 @@ bool rf_needs_windows(int cfg, int v_eff);
 // mode 0: meshes that fit the vertex cache; 1: any mesh (window by window
 // beyond the cache); 2: any mesh with the frame dump of the stage call.
 hipError_t render_fitness(int cfg, int mode, const QpsoSolveDev* d_solves, const QpsoItem* d_items, int n_items,
-                          hipStream_t st);
+                          hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 // argmin + pbest/gbest + particle update, one workgroup per solve.
-hipError_t pso_step(const QpsoSolveDev* d_solves, int n_solves, hipStream_t st);
+hipError_t pso_step(const QpsoSolveDev* d_solves, int n_solves, hipStream_t st, hipEvent_t ev0 = nullptr,
+                    hipEvent_t ev1 = nullptr);
 
 }  // namespace qpso_launch
 

@@ -15,6 +15,7 @@
 //                     (main.cu:1691-1698), UpdatePersonalAndGlobalBest
 //                     (main.cu:768-855) and UpdateParticlesEquationKernel
 //                     (main.cu:858-1060), with no host round trip.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "qpso_internal.h"
@@ -1251,36 +1252,55 @@ int rf_config_threads(int cfg) { return kRfCfgs[cfg].bt; }
 int rf_config_tile(int cfg) { return kRfCfgs[cfg].cap; }
 bool rf_needs_windows(int cfg, int v_eff) { return v_eff > kRfCfgs[cfg].cap / 6; }
 
-hipError_t init(const QpsoSolveDev* d_solves, int n_solves, hipStream_t st) {
+hipError_t init(const QpsoSolveDev* d_solves, int n_solves, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
     if (n_solves <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_init, dim3(n_solves), dim3(256), 0, st, d_solves);
+    if (ev0 || ev1)
+        hipExtLaunchKernelGGL(k_init, dim3(n_solves), dim3(256), 0, st, ev0, ev1, 0, d_solves);
+    else
+        hipLaunchKernelGGL(k_init, dim3(n_solves), dim3(256), 0, st, d_solves);
     return hipGetLastError();
 }
 
+// With events, the launch stamps them with the dispatch's own start and end
+// (hipExtLaunchKernel), so a timed kernel carries no event-record gaps.
+template <int BT, int CAP, int MODE>
+static void launch_rf_mode(const QpsoSolveDev* d_solves, const QpsoItem* d_items, int n_items, hipStream_t st,
+                           hipEvent_t ev0, hipEvent_t ev1) {
+    if (ev0 || ev1)
+        hipExtLaunchKernelGGL((k_render_fitness<BT, CAP, MODE>), dim3(n_items), dim3(BT), 0, st, ev0, ev1, 0,
+                              d_solves, d_items);
+    else
+        hipLaunchKernelGGL((k_render_fitness<BT, CAP, MODE>), dim3(n_items), dim3(BT), 0, st, d_solves, d_items);
+}
+
 template <int BT, int CAP>
-static void launch_rf(int mode, const QpsoSolveDev* d_solves, const QpsoItem* d_items, int n_items, hipStream_t st) {
-    if (mode == 2) hipLaunchKernelGGL((k_render_fitness<BT, CAP, 2>), dim3(n_items), dim3(BT), 0, st, d_solves, d_items);
-    else if (mode == 1) hipLaunchKernelGGL((k_render_fitness<BT, CAP, 1>), dim3(n_items), dim3(BT), 0, st, d_solves, d_items);
-    else hipLaunchKernelGGL((k_render_fitness<BT, CAP, 0>), dim3(n_items), dim3(BT), 0, st, d_solves, d_items);
+static void launch_rf(int mode, const QpsoSolveDev* d_solves, const QpsoItem* d_items, int n_items, hipStream_t st,
+                      hipEvent_t ev0, hipEvent_t ev1) {
+    if (mode == 2) launch_rf_mode<BT, CAP, 2>(d_solves, d_items, n_items, st, ev0, ev1);
+    else if (mode == 1) launch_rf_mode<BT, CAP, 1>(d_solves, d_items, n_items, st, ev0, ev1);
+    else launch_rf_mode<BT, CAP, 0>(d_solves, d_items, n_items, st, ev0, ev1);
 }
 
 hipError_t render_fitness(int cfg, int mode, const QpsoSolveDev* d_solves, const QpsoItem* d_items, int n_items,
-                          hipStream_t st) {
+                          hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
     if (n_items <= 0) return hipSuccess;
     switch (cfg) {
-        case 0: launch_rf<1024, 40448>(mode, d_solves, d_items, n_items, st); break;
-        case 1: launch_rf<512, 40448>(mode, d_solves, d_items, n_items, st); break;
-        case 2: launch_rf<512, 20224>(mode, d_solves, d_items, n_items, st); break;
-        case 3: launch_rf<256, 20224>(mode, d_solves, d_items, n_items, st); break;
-        case 4: launch_rf<256, 10112>(mode, d_solves, d_items, n_items, st); break;
+        case 0: launch_rf<1024, 40448>(mode, d_solves, d_items, n_items, st, ev0, ev1); break;
+        case 1: launch_rf<512, 40448>(mode, d_solves, d_items, n_items, st, ev0, ev1); break;
+        case 2: launch_rf<512, 20224>(mode, d_solves, d_items, n_items, st, ev0, ev1); break;
+        case 3: launch_rf<256, 20224>(mode, d_solves, d_items, n_items, st, ev0, ev1); break;
+        case 4: launch_rf<256, 10112>(mode, d_solves, d_items, n_items, st, ev0, ev1); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t pso_step(const QpsoSolveDev* d_solves, int n_solves, hipStream_t st) {
+hipError_t pso_step(const QpsoSolveDev* d_solves, int n_solves, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
     if (n_solves <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_pso_step<256>, dim3(n_solves), dim3(256), 0, st, d_solves);
+    if (ev0 || ev1)
+        hipExtLaunchKernelGGL(k_pso_step<256>, dim3(n_solves), dim3(256), 0, st, ev0, ev1, 0, d_solves);
+    else
+        hipLaunchKernelGGL(k_pso_step<256>, dim3(n_solves), dim3(256), 0, st, d_solves);
     return hipGetLastError();
 }
 

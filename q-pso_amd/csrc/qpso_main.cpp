@@ -8,8 +8,13 @@
 // `<depth_image>.labels.dpt` (uint16, label k+1 = cluster k), the place the
 // reference runs GraphCannySeg; rgb_image only names the outputs, as the
 // segmentation that would read it stays outside this hot path.  Outputs match
-// main.cu:1749-1762: results/<rgb>-<N>-bestpose.csv and results/<rgb>-<N>-best.csv.
-// Optional environment: QPSO_PARTICLES, QPSO_ITERS, QPSO_FX/FY/CX/CY.
+// main.cu:1749-1780: results/<rgb>-<N>-bestpose.csv, results/<rgb>-<N>-best.csv
+// and the CPU renders of the first 10 particles' final poses, 0.csv .. 9.csv.
+// Like the reference's 6x512-thread render grid (main.h:162-163) it renders
+// faces 0..3071 only; QPSO_FACE_CAP overrides that (0: every face).
+// Optional environment: QPSO_PARTICLES, QPSO_ITERS, QPSO_FX/FY/CX/CY,
+// QPSO_FACE_CAP.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -57,6 +62,7 @@ int main(int argc, char** argv) {
     p.cam.fy = env_d("QPSO_FY", p.cam.fy);
     p.cam.cx = env_d("QPSO_CX", p.cam.cx);
     p.cam.cy = env_d("QPSO_CY", p.cam.cy);
+    p.face_cap = (int)env_d("QPSO_FACE_CAP", 3072);  // fxptBlocks * fxptThsPerBlock
 
     std::vector<uint16_t> cdepth(QPSO_ROWS * QPSO_COLS);
     qpso_cluster cl{};
@@ -75,8 +81,17 @@ int main(int argc, char** argv) {
     }
     uint32_t mid = 0;
     qpso_result res{};
+    qpso_solve* solve = nullptr;
+    std::vector<float> pos(7 * (size_t)std::max(p.n_particles, 0));
     int r = qpso_mesh_upload(ctx, mesh, F, &mid);
-    if (!r) r = qpso_estimate_pose(ctx, mid, &cl, &p, &res);
+    if (!r) r = qpso_solve_create(ctx, mid, &cl, &p, 0, &solve);
+    if (!r) r = qpso_solve_run(solve, &res);
+    if (!r) {
+        qpso_trace tr{};
+        tr.pos = pos.data();  // d_pso_pos after the last iteration (main.cu:1764)
+        r = qpso_solve_get_trace(solve, &tr);
+    }
+    if (solve) qpso_solve_destroy(solve);
     if (r) {
         std::fprintf(stderr, "estimate_pose: %s (%s)\n", qpso_status_str(r), qpso_last_error(ctx));
         qpso_destroy(ctx);
@@ -87,6 +102,12 @@ int main(int argc, char** argv) {
     const std::string base = "results/" + rgb_name + "-" + std::to_string(idx);
     qpso_write_bestpose((base + "-bestpose.csv").c_str(), res.fit, res.pose);
     qpso_render_cpu_csv((base + "-best.csv").c_str(), res.pose, mesh, F, &p.cam);
+    // the first 10 particles' final poses (main.cu:1766-1780)
+    for (int k = 0; k < 10 && k < p.n_particles; ++k) {
+        float pose[7];
+        for (int d = 0; d < 7; ++d) pose[d] = pos[(size_t)d * p.n_particles + k];
+        qpso_render_cpu_csv((std::to_string(k) + ".csv").c_str(), pose, mesh, F, &p.cam);
+    }
     qpso_destroy(ctx);
     qpso_free(mesh);
     qpso_free(depth);

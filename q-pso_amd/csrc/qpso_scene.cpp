@@ -15,6 +15,8 @@
 #include <map>
 #include <vector>
 
+#include "qpso_roi.h"
+
 namespace {
 
 constexpr int kRows = 480;
@@ -249,8 +251,8 @@ int qpso_scene_render(const qpso_scene_object* objs, int n_objs, double fx, doub
 int qpso_scene_cluster(const uint16_t* depth, const int32_t* label, int obj, double fx, double fy,
                        double cx, double cy, qpso_cluster_roi* roi, uint16_t* cluster_depth_out) {
     int minx = kCols, miny = kRows, maxx = -1, maxy = -1;
-    uint32_t n = 0, nz = 0;
-    double sx = 0, sy = 0, sz = 0;
+    uint32_t n = 0;
+    qpso_roi::Centroid cen(qpso_roi::kinv(fx, fy, cx, cy));
     for (int y = 0; y < kRows; ++y)
         for (int x = 0; x < kCols; ++x) {
             const int i = y * kCols + x;
@@ -260,16 +262,10 @@ int qpso_scene_cluster(const uint16_t* depth, const int32_t* label, int obj, dou
             ++n;
             minx = std::min(minx, x); maxx = std::max(maxx, x);
             miny = std::min(miny, y); maxy = std::max(maxy, y);
-            if (depth[i] == 0) continue;
-            const double z = depth[i];
-            sx += (x - cx) * z / fx; sy += (y - cy) * z / fy; sz += z;
-            ++nz;
+            cen.add(x, y, depth[i]);
         }
-    if (n == 0 || nz == 0) return -1;
+    if (n == 0 || !cen.result(roi->centroid_mm)) return -1;
     roi->rect[0] = minx; roi->rect[1] = miny; roi->rect[2] = maxx + 1; roi->rect[3] = maxy + 1;
-    roi->centroid_mm[0] = (float)(sx / nz);
-    roi->centroid_mm[1] = (float)(sy / nz);
-    roi->centroid_mm[2] = (float)(sz / nz);
     roi->num_points = n;
     return 0;
 }

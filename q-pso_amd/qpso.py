@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libqpso.so")
 ROWS, COLS = 480, 640
 
 OK, E_INVAL, E_HIP, E_NOMEM, E_BUDGET, E_NOTFOUND, E_STALE, E_PENDING = 0, -1, -2, -3, -4, -5, -6, -7
+E_CANCELLED, E_TIMEOUT = -8, -9
 
 # Symbols every header in include/ declares (checked by the CPU test suite).
 EXPORTS = [
@@ -26,7 +27,8 @@ EXPORTS = [
     "qpso_solve_run_reference_launches",
     "qpso_solve_run_batch", "qpso_autotune", "qpso_set_tuning",
     "qpso_solve_eval_poses", "qpso_server_create", "qpso_server_set_slots", "qpso_server_submit", "qpso_server_poll",
-    "qpso_server_wait", "qpso_server_stats_get", "qpso_server_destroy", "qpso_solve_bytes",
+    "qpso_server_wait", "qpso_server_cancel", "qpso_server_set_timeout", "qpso_server_stats_get",
+    "qpso_server_destroy", "qpso_solve_bytes",
     "qpso_mesh_icosphere", "qpso_mesh_uv_ellipsoid", "qpso_random_quat", "qpso_scene_render",
     "qpso_scene_cluster", "qpso_read_obj", "qpso_read_dpt", "qpso_write_dpt", "qpso_free",
     "qpso_write_bestpose", "qpso_render_cpu", "qpso_render_cpu_csv", "qpso_clusters_from_labels",
@@ -52,7 +54,8 @@ class Cluster(C.Structure):
 
 class Result(C.Structure):
     _fields_ = [("pose", C.c_float * 7), ("fit", C.c_float), ("raster_px", C.c_uint64),
-                ("visit_px", C.c_uint64), ("n_degenerate", C.c_uint32), ("status", C.c_int32)]
+                ("visit_px", C.c_uint64), ("n_degenerate", C.c_uint32), ("status", C.c_int32),
+                ("latency_ms", C.c_double)]
 
 
 class Trace(C.Structure):
@@ -73,7 +76,8 @@ class Timing(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
                 ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double),
-                ("solves_per_s", C.c_double), ("steps", C.c_uint64), ("mean_active", C.c_double)]
+                ("solves_per_s", C.c_double), ("steps", C.c_uint64), ("mean_active", C.c_double),
+                ("cancelled", C.c_uint64), ("timed_out", C.c_uint64)]
 
 
 class SceneObject(C.Structure):
@@ -124,6 +128,8 @@ def lib():
             "qpso_server_submit": (I32, [P, U32, P, P, P]),
             "qpso_server_poll": (I32, [P, U64, P]),
             "qpso_server_wait": (I32, [P, U64, P]),
+            "qpso_server_cancel": (I32, [P, U64]),
+            "qpso_server_set_timeout": (I32, [P, C.c_double]),
             "qpso_server_stats_get": (I32, [P, P]),
             "qpso_server_destroy": (I32, [P]),
             "qpso_solve_bytes": (U64, [P, P]),
@@ -340,6 +346,18 @@ class Server:
             return None
         _check(rc)
         return r
+
+    def wait_status(self, ticket):
+        """(status, result) without raising on a per-request error."""
+        r = Result()
+        rc = lib().qpso_server_wait(self.handle, ticket, C.byref(r))
+        return rc, r
+
+    def cancel(self, ticket):
+        _check(lib().qpso_server_cancel(self.handle, ticket))
+
+    def set_timeout(self, timeout_ms):
+        _check(lib().qpso_server_set_timeout(self.handle, float(timeout_ms)))
 
     def stats(self):
         s = ServerStats()

@@ -7,12 +7,15 @@ each object (rect, centroid, pixel count, masked depth).  Meshes are handed
 to the solver in metres, like the reference's loadOBJ3 output.
 
   config0  icosphere (1,280 faces, 100x70x50 mm), N 32, I 30, one cluster
-  config1  UV ellipsoid 80x65 (10,240 faces) + 8 boxes, ROI ~220^2, N 256, I 100
+  config1  UV ellipsoid 80x65 (10,240 faces) + 8 boxes, ROI 220x220, N 256, I 100
   config2  16 objects in one frame, faces log-uniform [500, 20000],
            N in {64,128,256,512}, I 60, ROI 40^2..250^2
   config3  serving stream: 8 resident meshes (1k..50k faces), frames of 4..16
            clusters, N 128, I 60, Poisson arrivals
-  config4  100k-face mesh, ROI up to 320^2, N 2048, I 200, sigma 4, 15% holes
+  config4  100k-face mesh, ROI up to 320^2, N 2048, I 200, sigma 4, 15% holes;
+           config4_mix: 64 of those solves mixed with 64 1k-face/N 64 solves
+  reference_config  the reference's own defaults (N 1024, I 20, ACCV camera,
+           3072-face cap) on the config1 object
 """
 from __future__ import annotations
 
@@ -65,7 +68,8 @@ def config0(seed=0):
 
 def config1(seed=0, n_particles=256, n_iters=100):
     rng = np.random.default_rng(1000 + seed)
-    mesh = qpso.mesh_uv_ellipsoid(80, 65, 160.0, 115.0, 85.0, 8, 10.0, 20.0, seed + 7)
+    # semi-axes chosen so that at this pose the cluster rect is 220x220 px
+    mesh = qpso.mesh_uv_ellipsoid(80, 65, 164.0, 168.0, 162.0, 8, 10.0, 20.0, seed + 7)
     pose = gt_pose(rng, xy=60.0, z=(780.0, 820.0), seed=1000 + seed)
     depth, labels = qpso.scene_render([(mesh, pose)], CAM, 1100.0, 2.0, 0.05, seed + 2)
     scene = Scene([(mesh, pose)], depth, labels)
@@ -151,6 +155,48 @@ def config4(seed=0, n_particles=2048, n_iters=200):
     depth, labels = qpso.scene_render([(mesh, pose)], CAM, 1100.0, 4.0, 0.15, seed + 4)
     scene = Scene([(mesh, pose)], depth, labels)
     params = qpso.make_params(n_particles, n_iters, w=0.7, c1=1.5, c2=1.5, cam=CAM, seed=0)
+    return scene, params
+
+
+def config4_mix(seed=0, n_stress=64, n_small=64, stress_iters=200):
+    """The stress load: `n_stress` solves of the 100k-face scene (2048
+    particles, `stress_iters` iterations, each with its own random-table seed)
+    mixed with `n_small` solves of a 1k-face object (64 particles, 20..200
+    iterations).  Returns (stress scene, small scene, [(scene index, params)])
+    in submission order: the kinds interleaved."""
+    big, _ = config4(seed)
+    rng = np.random.default_rng(4400 + seed)
+    sl, st = uv_dims(1000 - 24)
+    mesh = qpso.mesh_uv_ellipsoid(sl, st, 40.0, 28.0, 20.0, 2, 4.0, 8.0, seed + 44)
+    pose = gt_pose(rng, xy=80.0, z=(600.0, 900.0), seed=4401 + seed)
+    depth, labels = qpso.scene_render([(mesh, pose)], CAM, 1100.0, 4.0, 0.15, seed + 45)
+    small = Scene([(mesh, pose)], depth, labels)
+    jobs = []
+    for k in range(max(n_stress, n_small)):
+        if k < n_stress:
+            jobs.append((0, qpso.make_params(2048, stress_iters, w=0.7, c1=1.5, c2=1.5, cam=CAM, seed=1 + k)))
+        if k < n_small:
+            iters = int(rng.integers(20, 201))
+            jobs.append((1, qpso.make_params(64, iters, w=0.7, c1=1.5, c2=1.5, cam=CAM, seed=1000 + k)))
+    return big, small, jobs
+
+
+# FX, FY, CX, CY of the reference (main.h:30-33)
+ACCV_CAM = (572.41140, 573.57043, 325.26110, 242.04899)
+
+
+def reference_config(seed=0):
+    """The reference's own run: qpso_params_default (N 1024, I 20, w 0.3,
+    c1 = c2 = 1, the ACCV intrinsics) with its 6 x 512-thread render grid,
+    i.e. faces 0..3071 only (face_cap 3072), on the config1 object seen
+    through the ACCV camera."""
+    rng = np.random.default_rng(1000 + seed)
+    mesh = qpso.mesh_uv_ellipsoid(80, 65, 164.0, 168.0, 162.0, 8, 10.0, 20.0, seed + 7)
+    pose = gt_pose(rng, xy=60.0, z=(780.0, 820.0), seed=1000 + seed)
+    depth, labels = qpso.scene_render([(mesh, pose)], ACCV_CAM, 1100.0, 2.0, 0.05, seed + 2)
+    scene = Scene([(mesh, pose)], depth, labels, cam=ACCV_CAM)
+    params = qpso.default_params()
+    params.face_cap = 3072
     return scene, params
 
 

@@ -148,3 +148,96 @@ def test_server_uses_context_tuning(ctx, frame):
             sv.close()
     finally:
         ctx.set_tuning(0, 96)
+
+
+@pytest.fixture(scope="module")
+def uneven_iters(ctx, frame):
+    """The frame's solves with different iteration counts (5..17), and their plain results."""
+    jobs, _ = frame
+    out, plain = [], []
+    for k, (mid, cl, p) in enumerate(jobs):
+        q = qpso.make_params(p.n_particles, 5 + 3 * k, w=p.w, c1=p.c1, c2=p.c2,
+                             cam=(p.cam.fx, p.cam.fy, p.cam.cx, p.cam.cy), seed=p.seed)
+        out.append((mid, cl, q))
+        with ctx.solve(mid, cl, q) as s:
+            plain.append(s.run())
+    return out, plain
+
+
+def test_batch_uneven_iterations_equals_plain(ctx, uneven_iters):
+    jobs, plain = uneven_iters
+    solves = [ctx.solve(mid, cl, p) for mid, cl, p in jobs]
+    try:
+        res = ctx.run_batch(solves)
+        for k, (a, b) in enumerate(zip(res, plain)):
+            assert same(a, b), f"solve {k} ({jobs[k][2].n_iters} iterations)"
+    finally:
+        for s in solves:
+            s.close()
+
+
+def test_server_uneven_iterations_equals_plain(ctx, uneven_iters):
+    jobs, plain = uneven_iters
+    sv = qpso.Server(ctx, 8 << 30)
+    try:
+        tickets = [(k, sv.submit(*jobs[k])) for k in range(len(jobs))]
+        for k, t in tickets:
+            assert same(sv.wait(t), plain[k]), f"request for solve {k}"
+    finally:
+        sv.close()
+
+
+def test_server_cancel_timeout_and_tickets(ctx, frame):
+    jobs, plain = frame
+    mid, cl, p = jobs[0]
+    long_p = qpso.make_params(p.n_particles, 2000, w=p.w, c1=p.c1, c2=p.c2,
+                              cam=(p.cam.fx, p.cam.fy, p.cam.cx, p.cam.cy), seed=p.seed)
+    sv = qpso.Server(ctx, 8 << 30)
+    try:
+        sv.set_slots(p.n_particles)  # one resident at a time: the second request queues
+        resident = sv.submit(mid, cl, long_p)
+        queued = sv.submit(mid, cl, long_p)
+        time.sleep(0.05)
+        sv.cancel(queued)
+        rc, r = sv.wait_status(queued)
+        assert rc == qpso.E_CANCELLED and r.status == qpso.E_CANCELLED
+        sv.cancel(resident)  # leaves at the next step boundary
+        rc, r = sv.wait_status(resident)
+        assert rc == qpso.E_CANCELLED
+        # a normal request afterwards is unaffected; its ticket is read once
+        t = sv.submit(mid, cl, p)
+        assert same(sv.wait(t), plain[0])
+        rc, _ = sv.wait_status(t)
+        assert rc == qpso.E_NOTFOUND
+        assert lib_rc(sv, t) == qpso.E_NOTFOUND
+        # time-out: requests submitted from now on end after 20 ms
+        sv.set_timeout(20.0)
+        t = sv.submit(mid, cl, long_p)
+        rc, _ = sv.wait_status(t)
+        assert rc == qpso.E_TIMEOUT
+        sv.set_timeout(0.0)
+        st = sv.stats()
+        assert st.cancelled == 2 and st.timed_out == 1 and st.completed == 1
+    finally:
+        sv.close()
+
+
+def lib_rc(sv, ticket):
+    return qpso.lib().qpso_server_poll(sv.handle, ticket, None)
+
+
+def test_server_destroy_releases_waiter(ctx, frame):
+    import threading
+    jobs, _ = frame
+    mid, cl, p = jobs[0]
+    long_p = qpso.make_params(p.n_particles, 2000, w=p.w, c1=p.c1, c2=p.c2,
+                              cam=(p.cam.fx, p.cam.fy, p.cam.cx, p.cam.cy), seed=p.seed)
+    sv = qpso.Server(ctx, 8 << 30)
+    t = sv.submit(mid, cl, long_p)
+    got = []
+    th = threading.Thread(target=lambda: got.append(sv.wait_status(t)[0]))
+    th.start()
+    time.sleep(0.05)
+    sv.close()  # returns once the blocked waiter has left
+    th.join(timeout=30)
+    assert not th.is_alive() and got == [qpso.E_CANCELLED]

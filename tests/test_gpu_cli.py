@@ -1,6 +1,6 @@
 """The drop-in CLI (`qpso_main obj rgb depth N`, the reference's `main`)
-against the library's per-cluster call: the same files in, the same
-bestpose.csv out."""
+against the oracle: the same files in, the oracle's best pose in
+bestpose.csv, and its final particles in 0.csv .. 9.csv."""
 import ctypes as C
 import os
 import subprocess
@@ -8,6 +8,7 @@ import subprocess
 import numpy as np
 import pytest
 
+import oracle as O
 import qpso
 import workloads as W
 
@@ -37,8 +38,26 @@ def read_obj(path):
     return out
 
 
-def test_cli_matches_estimate_pose(tmp_path):
-    scene, _ = W.config0(0)
+def oracle_cluster_params(labels, depth, k, cam, n, iters, face_cap):
+    """The cluster fields the CLI takes from the label map, and the oracle's
+    parameters for the same solve (bounds as main.cu:1284-1331)."""
+    L = qpso.lib()
+    p = qpso.default_params()
+    p.cam = qpso.Camera(*[float(x) for x in cam])
+    rect, cen, npts = (C.c_int32 * 4)(), (C.c_float * 3)(), C.c_uint32()
+    cdepth = np.zeros((480, 640), np.uint16)
+    assert L.qpso_clusters_from_labels(labels.ctypes.data, depth.ctypes.data, k, C.byref(p.cam), rect, cen,
+                                       C.byref(npts), cdepth.ctypes.data) == 0
+    c = [np.float32(x) * np.float32(0.001) for x in cen]
+    tb = np.float32(0.3)
+    op = O.make_params(n, iters, 0.3, 1.0, 1.0, [c[0] - tb, c[1] - tb, c[2] - tb], [c[0] + tb, c[1] + tb, c[2] + tb],
+                       list(rect), npts.value, cam, face_cap)
+    return op, cdepth
+
+
+def test_cli_matches_oracle(tmp_path):
+    # the 10,336-face scene: the CLI keeps the reference's 3072-face cap
+    scene, _ = W.config1(0)
     obj, dpt = tmp_path / "model.obj", tmp_path / "frame.dpt"
     write_obj(obj, scene.mesh_m())
     mesh = read_obj(obj)
@@ -50,28 +69,38 @@ def test_cli_matches_estimate_pose(tmp_path):
     assert L.qpso_write_dpt((str(dpt) + ".labels.dpt").encode(), labels.ctypes.data, 480, 640) == 0
     (tmp_path / "results").mkdir()
     cam = W.CAM
-    env = dict(os.environ, QPSO_PARTICLES="32", QPSO_ITERS="5", QPSO_FX=repr(cam[0]), QPSO_FY=repr(cam[1]),
-               QPSO_CX=repr(cam[2]), QPSO_CY=repr(cam[3]))
+    n, iters = 64, 5
+    env = dict(os.environ, QPSO_PARTICLES=str(n), QPSO_ITERS=str(iters), QPSO_FX=repr(cam[0]),
+               QPSO_FY=repr(cam[1]), QPSO_CX=repr(cam[2]), QPSO_CY=repr(cam[3]))
+    env.pop("QPSO_FACE_CAP", None)
+    print("cli: config1 mesh, 64 particles, 5 iterations", flush=True)
     run = subprocess.run([CLI, str(obj), "rgb/scene.png", str(dpt), "0"], cwd=tmp_path, env=env,
                          capture_output=True, text=True, timeout=120)
     assert run.returncode == 0, run.stderr
     assert "Solution Best Pose" in run.stdout
     got = (tmp_path / "results" / "scene.png-0-bestpose.csv").read_text()
 
-    # the same call through the library
-    p = qpso.default_params()
-    p.n_particles, p.n_iters = 32, 5
-    p.cam = qpso.Camera(*[float(x) for x in cam])
-    rect, cen, npts = (C.c_int32 * 4)(), (C.c_float * 3)(), C.c_uint32()
-    cdepth = np.zeros((480, 640), np.uint16)
-    assert L.qpso_clusters_from_labels(labels.ctypes.data, depth.ctypes.data, 0, C.byref(p.cam), rect, cen,
-                                       C.byref(npts), cdepth.ctypes.data) == 0
-    cl = qpso.make_cluster(list(rect), list(cen), npts.value, cdepth)
-    with qpso.Context(0) as ctx:
-        mid = ctx.upload_mesh(mesh)
-        res = ctx.estimate_pose(mid, cl, p)
-        ctx.release_mesh(mid)
+    # the same solve on the oracle
+    op, cdepth = oracle_cluster_params(labels, depth, 0, cam, n, iters, 3072)
+    rand = O.rand_table(0, n, iters)
+    o = O.solve(mesh, cdepth, op, rand, n_threads=16)
     ref = tmp_path / "ref.csv"
-    pose = np.array(res.pose[:], np.float32)
-    assert L.qpso_write_bestpose(str(ref).encode(), C.c_float(res.fit), pose.ctypes.data) == 0
+    assert L.qpso_write_bestpose(str(ref).encode(), C.c_float(o["best_fit"]), o["best_pose"].ctypes.data) == 0
     assert got == ref.read_text()
+    # the first 10 particles' final poses rendered on the CPU (main.cu:1766-1780)
+    camc = qpso.Camera(*[float(x) for x in cam])
+    for k in range(10):
+        pose = np.ascontiguousarray(o["pos"][:, k])
+        want = tmp_path / f"want{k}.csv"
+        assert L.qpso_render_cpu_csv(str(want).encode(), pose.ctypes.data, mesh.ctypes.data, mesh.size // 9,
+                                     C.byref(camc)) == 0
+        assert (tmp_path / f"{k}.csv").read_text() == want.read_text(), f"particle {k}"
+    # without the cap (QPSO_FACE_CAP=0) the solve differs: faces beyond 3072 are drawn
+    env["QPSO_FACE_CAP"] = "0"
+    run = subprocess.run([CLI, str(obj), "rgb/scene.png", str(dpt), "0"], cwd=tmp_path, env=env,
+                         capture_output=True, text=True, timeout=120)
+    assert run.returncode == 0, run.stderr
+    op, _ = oracle_cluster_params(labels, depth, 0, cam, n, iters, 0)
+    o = O.solve(mesh, cdepth, op, rand, n_threads=16)
+    assert L.qpso_write_bestpose(str(ref).encode(), C.c_float(o["best_fit"]), o["best_pose"].ctypes.data) == 0
+    assert (tmp_path / "results" / "scene.png-0-bestpose.csv").read_text() == ref.read_text()

@@ -41,8 +41,8 @@ def test_params_default_are_reference_constants():
 
 def test_status_strings():
     L = qpso.lib()
-    for code in (0, -1, -2, -3, -4, -5, -6, -7):
-        assert L.qpso_status_str(code)
+    for code in (0, -1, -2, -3, -4, -5, -6, -7, -8, -9):
+        assert L.qpso_status_str(code) != b"unknown status"
     assert L.qpso_status_str(-99) == b"unknown status"
 
 
@@ -184,3 +184,139 @@ def test_clusters_from_label_map():
     assert out.sum() == 599 * 1000 and abs(cen[2] - 1000) < 1e-3
     assert L.qpso_clusters_from_labels(labels.ctypes.data, depth.ctypes.data, 0, C.byref(cam), rect, cen,
                                        C.byref(npts), out.ctypes.data) == qpso.E_NOTFOUND
+
+
+# ---------------------------------------------------------------- cluster centroid
+def centroid_restated(labels_mask, depth, cam):
+    """compute3DCentroid (GraphCannySeg.h:2569-2597) with projectPixel2CameraRF
+    (GraphCannySeg.h:2510-2517): pixels with depth z != 0 give the float
+    vector (u z, v z, z) times K^-1, whose entries are OpenCV's 3x3 cofactor
+    inverse of the float camera matrix (double, rounded once per entry) and
+    whose rows are summed in double and rounded once; the points are summed
+    in float in row-major order and divided by their number in float."""
+    fx, fy, cx, cy = (np.float64(np.float32(c)) for c in cam)
+    det = fx * fy
+    k00, k02 = np.float32(fy / det), np.float32(-(cx * fy) / det)
+    k11, k12 = np.float32(fx / det), np.float32(-(fx * cy) / det)
+    v, u = np.nonzero(labels_mask & (depth != 0))  # row-major order
+    z = depth[v, u].astype(np.int64)
+    p0, p1, p2 = (u * z).astype(np.float32), (v * z).astype(np.float32), z.astype(np.float32)
+    X = (np.float64(k00) * p0 + np.float64(k02) * p2).astype(np.float32)
+    Y = (np.float64(k11) * p1 + np.float64(k12) * p2).astype(np.float32)
+    n = np.float32(len(z))
+    return [np.cumsum(a, dtype=np.float32)[-1] / n for a in (X, Y, p2)]
+
+
+def test_centroid_matches_graphcannyseg_restatement():
+    rng = np.random.default_rng(4)
+    cam = (572.41140, 573.57043, 325.26110, 242.04899)
+    L = qpso.lib()
+    for trial in range(4):
+        labels = np.zeros((480, 640), np.uint16)
+        y0, x0 = rng.integers(0, 300), rng.integers(0, 400)
+        h, w = rng.integers(20, 170), rng.integers(20, 230)
+        labels[y0:y0 + h, x0:x0 + w] = 1
+        labels[rng.random((480, 640)) < 0.02] = 0  # ragged mask
+        depth = rng.integers(300, 4000, (480, 640)).astype(np.uint16)
+        depth[rng.random((480, 640)) < 0.1] = 0  # holes: counted in num_points, not in the centroid
+        rect, cen, npts = (C.c_int32 * 4)(), (C.c_float * 3)(), C.c_uint32()
+        camc = qpso.Camera(*cam)
+        assert L.qpso_clusters_from_labels(labels.ctypes.data, depth.ctypes.data, 0, C.byref(camc), rect, cen,
+                                           C.byref(npts), None) == 0
+        want = centroid_restated(labels == 1, depth, cam)
+        assert np.array_equal(np.array(cen[:], np.float32).view(np.uint32), np.array(want, np.float32).view(np.uint32))
+        assert npts.value == int((labels == 1).sum())
+        # the synthetic scenes' provider is the same computation
+        c = qpso.scene_cluster(depth, np.where(labels == 1, 0, -1).astype(np.int32), 0, cam)
+        assert np.array_equal(np.array(c["centroid_mm"], np.float32).view(np.uint32),
+                              np.array(want, np.float32).view(np.uint32))
+        # and close to the exact mean back-projection
+        v, u = np.nonzero((labels == 1) & (depth != 0))
+        zz = depth[v, u].astype(np.float64)
+        exact = [np.mean((u - cam[2]) * zz / cam[0]), np.mean((v - cam[3]) * zz / cam[1]), zz.mean()]
+        assert np.allclose(want, exact, rtol=1e-4, atol=0.05)
+
+
+# ---------------------------------------------------------------- CPU visualisation render
+def render_cpu_restated(pose, soa, cam):
+    """RenderCPU + CPUfixedPointRender (main.h:1836-1991, 1993-2010): host
+    float arithmetic without contraction, iround(v) = (int)(v + 0.5f), depth
+    Z1 + cx2 z2 + cx3 z3 in float, kept where strictly below the buffer
+    (60000.0f).  Every edge value is evaluated directly at its pixel,
+    C + DX (16 y) - DY (16 x) mod 2^32, instead of the reference's
+    incremental walk; off-frame pixels are clipped."""
+    f32 = np.float32
+    F = soa.size // 9
+    buf = np.full((480, 640), f32(60000.0), np.float32)
+    t = [f32(x) for x in pose]
+    q0, q1, q2, q3 = t[3:]
+    for i in range(F):
+        U, V, Z = [], [], []
+        for j in range(3):
+            vx, vy, vz = f32(soa[i + j * F]), f32(soa[3 * F + i + j * F]), f32(soa[6 * F + i + j * F])
+            c = [q2 * vz - q3 * vy, -(q1 * vz - q3 * vx), q1 * vy - q2 * vx]
+            tt = [f32(2.0) * c[0], f32(2.0) * c[1], f32(2.0) * c[2]]
+            k = [q2 * tt[2] - q3 * tt[1], -(q1 * tt[2] - q3 * tt[0]), q1 * tt[1] - q2 * tt[0]]
+            px = vx + q0 * tt[0] + k[0] + t[0]
+            py = vy + q0 * tt[1] + k[1] + t[1]
+            pz = vz + q0 * tt[2] + k[2] + t[2]
+            r = f32(1.0) / pz
+            U.append(f32(cam[2] + cam[0] * float(px * r)))
+            V.append(f32(cam[3] + cam[1] * float(py * r)))
+            Z.append(f32(1000.0) * pz)
+        X = [int(f32(f32(16.0) * u) + f32(0.5)) for u in U]  # C truncation toward zero
+        Y = [int(f32(f32(16.0) * v) + f32(0.5)) for v in V]
+        area = (X[0] - X[1]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (Y[0] - Y[1])
+        if area <= 0:
+            continue
+        inv = f32(1.0) / f32(area)
+        minx, maxx = (min(X) + 15) >> 4, (max(X) + 15) >> 4
+        miny, maxy = (min(Y) + 15) >> 4, (max(Y) + 15) >> 4
+        if maxx - minx <= 0 or maxy - miny <= 0:
+            continue
+        yy, xx = np.mgrid[miny:maxy, minx:maxx].astype(np.int64)
+
+        def edge(a, b):
+            dx, dy = X[a] - X[b], Y[a] - Y[b]
+            c0 = dy * X[a] - dx * Y[a] + (1 if (dy < 0 or (dy == 0 and dx > 0)) else 0)
+            e = (c0 + dx * 16 * yy - dy * 16 * xx) & 0xFFFFFFFF
+            return np.where(e >= 2 ** 31, e - 2 ** 32, e)
+
+        cx2 = ((edge(1, 2) + 15) >> 8).astype(np.float32)
+        cx3 = ((edge(2, 0) + 15) >> 8).astype(np.float32)
+        z2, z3 = (Z[1] - Z[0]) * inv, (Z[2] - Z[0]) * inv
+        d = (Z[0] + cx2 * z2) + cx3 * z3
+        ok = (xx >= 0) & (xx < 640) & (yy >= 0) & (yy < 480)
+        for y, x, dv in zip(yy[ok], xx[ok], d[ok]):
+            if dv < buf[y, x]:
+                buf[y, x] = dv
+    return buf
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_render_cpu_matches_restatement(tmp_path, case):
+    rng = np.random.default_rng(50 + case)
+    soa = (qpso.mesh_icosphere(2, 60.0, 45.0, 35.0) * np.float32(1e-3)).astype(np.float32)
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    q = q if q[0] >= 0 else -q
+    # the last case straddles the frame's left edge
+    z = rng.uniform(0.5, 0.9)
+    tx = -325.26 / 572.41 * z if case == 3 else rng.uniform(-0.1, 0.1)
+    pose = np.array([tx, rng.uniform(-0.1, 0.1), z, *q], np.float32)
+    cam = qpso.Camera(572.41140, 573.57043, 325.26110, 242.04899)
+    buf = np.full((480, 640), np.float32(60000.0), np.float32)
+    L = qpso.lib()
+    assert L.qpso_render_cpu(pose.ctypes.data, soa.ctypes.data, soa.size // 9, C.byref(cam), buf.ctypes.data) == 0
+    want = render_cpu_restated(pose, soa, (cam.fx, cam.fy, cam.cx, cam.cy))
+    assert (want < 60000).sum() > 200
+    assert np.array_equal(buf.view(np.uint32), want.view(np.uint32))
+    # saveBestParticleRender2MatlabCSV (main.h:1993-2010): ostream default
+    # float format, ", " between columns, one row per line
+    path = tmp_path / "best.csv"
+    assert L.qpso_render_cpu_csv(str(path).encode(), pose.ctypes.data, soa.ctypes.data, soa.size // 9,
+                                 C.byref(cam)) == 0
+    lines = path.read_text().split("\n")
+    assert len(lines) == 481 and lines[-1] == ""
+    for r in range(0, 480, 7):
+        assert lines[r] == ", ".join(f"{float(v):g}" for v in want[r])
