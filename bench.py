@@ -194,7 +194,7 @@ def run_server(ctx, jobs, arrivals, slots, budget=32 << 30, warm_frac=0.0):
     return st, res, pct(lat, 0.5), pct(lat, 0.99)
 
 
-def served_section(ctx, lams=(200.0, 1000.0, 4000.0), seconds=2.5, seed=0, slots=8192):
+def served_section(ctx, lams=(200.0, 1000.0, 4000.0), seconds=2.5, seed=0, slots=8192, n_sat=1500):
     """(frame, cluster) requests into the continuous-batching server: the
     saturation capacity (every request submitted at once), then Poisson
     arrivals per rate; p50/p99 over requests submitted after the first 30 %
@@ -227,7 +227,6 @@ def served_section(ctx, lams=(200.0, 1000.0, 4000.0), seconds=2.5, seed=0, slots
         ctx.estimate_pose(mid, cl, params)
     plain_rate = n_plain / (time.perf_counter() - t)
     # saturation: every request at once
-    n_sat = 1500
     jobs = [(pool[k % len(pool)][0], pool[k % len(pool)][1], params) for k in range(n_sat)]
     st, _, _, _ = run_server(ctx, jobs, np.zeros(n_sat), slots)
     capacity = st.solves_per_s
@@ -292,6 +291,8 @@ def main():
     ap.add_argument("--no-stress", action="store_true")
     ap.add_argument("--lam", type=str, default="200,1000,4000", help="arrival rates of the served section")
     ap.add_argument("--slots", type=int, default=8192, help="server particle slots (particles rendered per step)")
+    ap.add_argument("--served-seconds", type=float, default=2.5, help="Poisson arrivals per rate (s)")
+    ap.add_argument("--sat-requests", type=int, default=1500, help="requests of the saturation run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -367,7 +368,8 @@ def main():
     if not args.no_batched:
         out["batched"] = batched_section(ctx)
     if not args.no_served:
-        out["served"] = served_section(ctx, lams=[float(x) for x in args.lam.split(",")], slots=args.slots)
+        out["served"] = served_section(ctx, lams=[float(x) for x in args.lam.split(",")], seconds=args.served_seconds,
+                                       slots=args.slots, n_sat=args.sat_requests)
     if not args.no_stress:
         out["stress"] = stress_section(ctx)
     ctx.close()

@@ -28,6 +28,7 @@
 #include "qpso_internal.h"
 #include "qpso_kernels.h"
 #include "qpso_sched.h"
+#include "qpso_tables.h"
 
 namespace {
 
@@ -40,37 +41,6 @@ constexpr size_t kMaxRand = size_t(1) << 30;  // random-table entries (4 GiB)
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-// ---- GSL mt19937 (main.cu:1340-1352): gsl_rng_default with seed 0 -> 4357,
-//      2002 init_genrand seeding, gsl_rng_uniform = u32 / 2^32 --------------
-struct Mt19937 {
-    uint32_t mt[624];
-    int mti;
-    explicit Mt19937(uint32_t s) {
-        if (s == 0) s = 4357u;
-        mt[0] = s;
-        for (int i = 1; i < 624; ++i) mt[i] = 1812433253u * (mt[i - 1] ^ (mt[i - 1] >> 30)) + (uint32_t)i;
-        mti = 624;
-    }
-    uint32_t next() {
-        if (mti >= 624) {
-            for (int k = 0; k < 624; ++k) {
-                const uint32_t y = (mt[k] & 0x80000000u) | (mt[(k + 1) % 624] & 0x7fffffffu);
-                mt[k] = mt[(k + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
-            }
-            mti = 0;
-        }
-        uint32_t y = mt[mti++];
-        y ^= y >> 11;
-        y ^= (y << 7) & 0x9d2c5680u;
-        y ^= (y << 15) & 0xefc60000u;
-        y ^= y >> 18;
-        return y;
-    }
-};
-
-int rand_stride(int iters) { return std::max(512, 12 * (iters + 1)); }
-// GSL replaces seed 0 by its default 4357 (gsl_rng_set): both name one table
-uint32_t rand_seed(uint32_t seed) { return seed == 0 ? 4357u : seed; }
 
 // One device random table, shared by the solves of equal (seed, N, S).
 struct RandTable {
@@ -100,90 +70,6 @@ void free_mesh(Mesh& m) {
     hipFree(m.corners);
     hipFree(m.runs);
     m = Mesh{};
-}
-
-// Deduplicate the corners of the non-indexed SoA mesh (bitwise-equal
-// coordinates become one vertex, numbered by first use in face order, so the
-// vertices of faces [0, f) are the prefix [0, v_prefix[f-1])) and build the
-// raster table in face order: neighbouring faces in one wave write
-// neighbouring pixels, which keeps ds_min_i32 free of bank conflicts.
-// Runs of consecutive faces whose corners span at most kRunVerts vertex
-// indices (the smallest projected-vertex cache of the kernels), greedy in face
-// order.  Vertices are numbered by first use, so neighbouring faces of an
-// ordinary mesh share a short index range.
-constexpr int32_t kRunVerts = 10112 / 6;
-void build_runs(const std::vector<int32_t>& tri, uint32_t F, std::vector<int32_t>& runs) {
-    runs.clear();
-    uint32_t f = 0;
-    while (f < F) {
-        const uint32_t start = f;
-        int32_t lo = INT32_MAX, hi = INT32_MIN;
-        for (; f < F; ++f) {
-            const int32_t* t = &tri[(size_t)f * 3];
-            const int32_t nlo = std::min(lo, std::min(t[0], std::min(t[1], t[2])));
-            const int32_t nhi = std::max(hi, std::max(t[0], std::max(t[1], t[2])));
-            if (f > start && nhi - nlo + 1 > kRunVerts) break;
-            lo = nlo;
-            hi = nhi;
-        }
-        if (hi - lo + 1 > kRunVerts) {  // one face alone spans more: no runs for this mesh
-            runs.clear();
-            return;
-        }
-        runs.push_back((int32_t)start);
-        runs.push_back(lo);
-        runs.push_back(hi + 1);
-    }
-}
-
-void build_mesh_tables(const float* A, uint32_t F, std::vector<float>& verts, std::vector<QpsoFaceRef>& order,
-                       std::vector<uint32_t>& v_prefix, std::vector<int32_t>& runs) {
-    struct Key {
-        uint32_t x, y, z;
-        bool operator==(const Key& o) const { return x == o.x && y == o.y && z == o.z; }
-    };
-    struct KeyHash {
-        size_t operator()(const Key& k) const {
-            uint64_t h = k.x * 0x9E3779B97F4A7C15ull;
-            h ^= (h >> 29) + k.y * 0xBF58476D1CE4E5B9ull;
-            h ^= (h >> 31) + k.z * 0x94D049BB133111EBull;
-            return (size_t)(h ^ (h >> 32));
-        }
-    };
-    std::unordered_map<Key, int32_t, KeyHash> ids;
-    ids.reserve((size_t)F * 2);
-    std::vector<int32_t> tri((size_t)F * 3);
-    std::vector<float> xs, ys, zs;
-    v_prefix.resize(F);
-    for (uint32_t f = 0; f < F; ++f) {
-        for (int j = 0; j < 3; ++j) {
-            const float x = A[f + j * (size_t)F], y = A[3 * (size_t)F + f + j * (size_t)F],
-                        z = A[6 * (size_t)F + f + j * (size_t)F];
-            Key k;
-            std::memcpy(&k.x, &x, 4); std::memcpy(&k.y, &y, 4); std::memcpy(&k.z, &z, 4);
-            auto it = ids.find(k);
-            int32_t id;
-            if (it == ids.end()) {
-                id = (int32_t)xs.size();
-                ids.emplace(k, id);
-                xs.push_back(x); ys.push_back(y); zs.push_back(z);
-            } else {
-                id = it->second;
-            }
-            tri[(size_t)f * 3 + j] = id;
-        }
-        v_prefix[f] = (uint32_t)xs.size();
-    }
-    const size_t V = xs.size();
-    verts.resize(3 * V);
-    std::copy(xs.begin(), xs.end(), verts.begin());
-    std::copy(ys.begin(), ys.end(), verts.begin() + V);
-    std::copy(zs.begin(), zs.end(), verts.begin() + 2 * V);
-    order.clear();
-    order.reserve(F);
-    for (uint32_t f = 0; f < F; ++f)
-        order.push_back(QpsoFaceRef{tri[f * 3], tri[f * 3 + 1], tri[f * 3 + 2], (int32_t)f});
-    build_runs(tri, F, runs);
 }
 
 struct ArenaLayout {
@@ -218,6 +104,57 @@ ArenaLayout layout(int N, int I, int rw, int rh, bool trace) {
     }
     L.total = o;
     return L;
+}
+
+// The device descriptor of a solve whose arena starts at A (layout L).
+void fill_desc(QpsoSolveDev& h, const Mesh& m, const qpso_cluster* cl, const qpso_params* p, char* A,
+               const ArenaLayout& L, const float* rand, bool trace, int big_face) {
+    h = QpsoSolveDev{};
+    const int N = p->n_particles, I = p->n_iters;
+    h.F = (int32_t)m.F;
+    h.F_eff = (p->face_cap > 0 && (uint32_t)p->face_cap < m.F) ? p->face_cap : (int32_t)m.F;
+    h.verts = m.verts;
+    h.V = (int32_t)m.V;
+    h.V_eff = (int32_t)m.v_prefix[h.F_eff - 1];
+    h.faces = m.faces;
+    h.corners = m.corners;
+    h.n_faces = (int32_t)m.F;
+    h.runs = m.runs;
+    h.n_runs = m.n_runs;
+    h.fx = p->cam.fx; h.fy = p->cam.fy; h.cx = p->cam.cx; h.cy = p->cam.cy;
+    h.N = N; h.I = I; h.S = qpso_host::rand_stride(I);
+    h.w = p->w; h.c1 = p->c1; h.c2 = p->c2;
+    // translation bounds around the centroid in metres (main.cu:1290-1331)
+    const float cxm = cl->centroid_mm[0] * 0.001f, cym = cl->centroid_mm[1] * 0.001f,
+                czm = cl->centroid_mm[2] * 0.001f;
+    h.x_lo[0] = cxm - p->t_bound;   h.x_hi[0] = cxm + p->t_bound;
+    h.x_lo[1] = cym - p->t_bound;   h.x_hi[1] = cym + p->t_bound;
+    h.x_lo[2] = czm - p->t_bound_z; h.x_hi[2] = czm + p->t_bound_z;
+    std::memcpy(h.rect, cl->rect, sizeof(h.rect));
+    h.num_points = cl->num_points;
+    h.obs = reinterpret_cast<const uint16_t*>(A + L.obs);
+    h.obs_rows = reinterpret_cast<const uint16_t*>(A + L.obs_rows);
+    h.rand = rand;
+    h.pos = reinterpret_cast<float*>(A + L.pos);
+    h.vel = reinterpret_cast<float*>(A + L.vel);
+    h.pbest = reinterpret_cast<float*>(A + L.pbest);
+    h.pbest_fit = reinterpret_cast<float*>(A + L.pbest_fit);
+    h.fit = reinterpret_cast<float*>(A + L.fit);
+    h.gbest = reinterpret_cast<float*>(A + L.gbest);
+    h.iter = reinterpret_cast<int32_t*>(A + L.iter);
+    h.stale[0] = reinterpret_cast<QpsoStale*>(A + L.stale0);
+    h.stale[1] = reinterpret_cast<QpsoStale*>(A + L.stale1);
+    h.stale_cnt[0] = reinterpret_cast<int32_t*>(A + L.cnt0);
+    h.stale_cnt[1] = reinterpret_cast<int32_t*>(A + L.cnt1);
+    h.stale_cap = kStaleCap;
+    h.big_face = big_face;
+    h.flags = reinterpret_cast<int32_t*>(A + L.flags);
+    h.counters = reinterpret_cast<unsigned long long*>(A + L.counters);
+    if (trace) {
+        h.fit_trace = reinterpret_cast<float*>(A + L.fit_tr);
+        h.gbest_trace = reinterpret_cast<float*>(A + L.gbest_tr);
+        h.argmin_trace = reinterpret_cast<int32_t*>(A + L.argmin_tr);
+    }
 }
 
 }  // namespace
@@ -301,9 +238,8 @@ int get_rand(qpso_ctx* ctx, const qpso_sched::TableKey& key, float** out) {
     const uint32_t seed = std::get<0>(key);
     const int N = std::get<1>(key), S = std::get<2>(key);
     if ((size_t)S * N > kMaxRand) return set_err(ctx, QPSO_E_NOMEM, "random table too large");
-    std::vector<float> h((size_t)S * N);
-    Mt19937 mt(seed);
-    for (auto& v : h) v = (float)((double)mt.next() / 4294967296.0);
+    std::vector<float> h;
+    qpso_host::rand_table(seed, (size_t)S * N, h);
     float* d = nullptr;
     HIP_TRY(ctx, hipMalloc(&d, h.size() * sizeof(float)));
     if (hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
@@ -505,18 +441,11 @@ int qpso_mesh_upload(qpso_ctx* ctx, const float* soa, uint32_t n_faces, uint32_t
     std::vector<float> verts;
     std::vector<QpsoFaceRef> order;
     std::vector<int32_t> runs;
-    build_mesh_tables(soa, n_faces, verts, order, m.v_prefix, runs);
+    qpso_host::build_mesh_tables(soa, n_faces, verts, order, m.v_prefix, runs);
     m.n_runs = (int)(runs.size() / 3);
     m.V = (uint32_t)(verts.size() / 3);
-    // corner table: (x, y, z) of corner j of face f, and its vertex index as the bits of w
-    std::vector<float> corners((size_t)n_faces * 12, 0.0f);
-    for (size_t f = 0; f < n_faces; ++f) {
-        const int32_t ids[3] = {order[f].v0, order[f].v1, order[f].v2};
-        for (int j = 0; j < 3; ++j) {
-            for (int c = 0; c < 3; ++c) corners[f * 12 + j * 4 + c] = soa[(size_t)c * 3 * n_faces + f + (size_t)j * n_faces];
-            std::memcpy(&corners[f * 12 + j * 4 + 3], &ids[j], sizeof(int32_t));
-        }
-    }
+    std::vector<float> corners;
+    qpso_host::build_corner_table(soa, n_faces, order, corners);
     hipError_t e = hipMalloc(&m.corners, sizeof(float) * corners.size());
     if (e == hipSuccess) e = hipMalloc(&m.verts, sizeof(float) * verts.size());
     if (e == hipSuccess) e = hipMalloc(&m.faces, sizeof(QpsoFaceRef) * order.size());
@@ -567,7 +496,7 @@ int qpso_solve_create(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl, c
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     auto mit = ctx->meshes.find(mesh_id);
     if (mit == ctx->meshes.end() || mit->second.released) return set_err(ctx, QPSO_E_NOTFOUND, "unknown mesh id");
-    const int N = p->n_particles, I = p->n_iters, S = rand_stride(I);
+    const int N = p->n_particles, I = p->n_iters, S = qpso_host::rand_stride(I);
     const int rw = cl->rect[2] - cl->rect[0], rh = cl->rect[3] - cl->rect[1];
 
     std::unique_ptr<qpso_solve> s(new qpso_solve());
@@ -580,70 +509,18 @@ int qpso_solve_create(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl, c
     if (hipMalloc(&s->arena, s->L.total) != hipSuccess)
         return set_err(ctx, QPSO_E_NOMEM, "solve arena allocation failed");
     float* rand = nullptr;
-    s->rand_key = qpso_sched::TableKey(rand_seed(p->seed), N, S);
+    s->rand_key = qpso_sched::TableKey(qpso_host::rand_seed(p->seed), N, S);
     if (int r = get_rand(ctx, s->rand_key, &rand)) { hipFree(s->arena); return r; }
 
-    QpsoSolveDev& h = s->h;
-    char* A = s->arena;
-    h.F = (int32_t)mit->second.F;
-    h.F_eff = (p->face_cap > 0 && (uint32_t)p->face_cap < mit->second.F) ? p->face_cap : (int32_t)mit->second.F;
-    h.verts = mit->second.verts;
-    h.V = (int32_t)mit->second.V;
-    h.V_eff = (int32_t)mit->second.v_prefix[h.F_eff - 1];
-    h.faces = mit->second.faces;
-    h.corners = mit->second.corners;
-    h.n_faces = (int32_t)mit->second.F;
-    h.runs = mit->second.runs;
-    h.n_runs = mit->second.n_runs;
-    h.fx = p->cam.fx; h.fy = p->cam.fy; h.cx = p->cam.cx; h.cy = p->cam.cy;
-    h.N = N; h.I = I; h.S = S;
-    h.w = p->w; h.c1 = p->c1; h.c2 = p->c2;
-    // translation bounds around the centroid in metres (main.cu:1290-1331)
-    const float cxm = cl->centroid_mm[0] * 0.001f, cym = cl->centroid_mm[1] * 0.001f,
-                czm = cl->centroid_mm[2] * 0.001f;
-    h.x_lo[0] = cxm - p->t_bound;   h.x_hi[0] = cxm + p->t_bound;
-    h.x_lo[1] = cym - p->t_bound;   h.x_hi[1] = cym + p->t_bound;
-    h.x_lo[2] = czm - p->t_bound_z; h.x_hi[2] = czm + p->t_bound_z;
-    std::memcpy(h.rect, cl->rect, sizeof(h.rect));
-    h.num_points = cl->num_points;
-    h.obs = reinterpret_cast<const uint16_t*>(A + s->L.obs);
-    h.obs_rows = reinterpret_cast<const uint16_t*>(A + s->L.obs_rows);
-    h.rand = rand;
-    h.pos = reinterpret_cast<float*>(A + s->L.pos);
-    h.vel = reinterpret_cast<float*>(A + s->L.vel);
-    h.pbest = reinterpret_cast<float*>(A + s->L.pbest);
-    h.pbest_fit = reinterpret_cast<float*>(A + s->L.pbest_fit);
-    h.fit = reinterpret_cast<float*>(A + s->L.fit);
-    h.gbest = reinterpret_cast<float*>(A + s->L.gbest);
-    h.iter = reinterpret_cast<int32_t*>(A + s->L.iter);
-    h.stale[0] = reinterpret_cast<QpsoStale*>(A + s->L.stale0);
-    h.stale[1] = reinterpret_cast<QpsoStale*>(A + s->L.stale1);
-    h.stale_cnt[0] = reinterpret_cast<int32_t*>(A + s->L.cnt0);
-    h.stale_cnt[1] = reinterpret_cast<int32_t*>(A + s->L.cnt1);
-    h.stale_cap = kStaleCap;
-    h.big_face = ctx->big_face;
+    fill_desc(s->h, mit->second, cl, p, s->arena, s->L, rand, s->trace, ctx->big_face);
     s->rf_cfg = ctx->rf_cfg;
-    h.flags = reinterpret_cast<int32_t*>(A + s->L.flags);
-    h.counters = reinterpret_cast<unsigned long long*>(A + s->L.counters);
-    if (s->trace) {
-        h.fit_trace = reinterpret_cast<float*>(A + s->L.fit_tr);
-        h.gbest_trace = reinterpret_cast<float*>(A + s->L.gbest_tr);
-        h.argmin_trace = reinterpret_cast<int32_t*>(A + s->L.argmin_tr);
-    }
+    char* A = s->arena;
     s->d_desc = reinterpret_cast<QpsoSolveDev*>(A + s->L.desc);
 
-    // cluster depth: only its rect is non-zero, keep that crop resident
-    std::vector<uint16_t> crop((size_t)rw * rh);
-    for (int y = 0; y < rh; ++y)
-        std::memcpy(&crop[(size_t)y * rw], cl->depth + (size_t)(cl->rect[1] + y) * kCols + cl->rect[0],
-                    sizeof(uint16_t) * rw);
-    // per crop row, the number of observed (non-zero) pixels left of each column
-    std::vector<uint16_t> rows((size_t)(rw + 1) * rh);
-    for (int y = 0; y < rh; ++y) {
-        uint16_t* r = &rows[(size_t)y * (rw + 1)];
-        r[0] = 0;
-        for (int x = 0; x < rw; ++x) r[x + 1] = (uint16_t)(r[x] + (crop[(size_t)y * rw + x] != 0 ? 1 : 0));
-    }
+    // cluster depth: only its rect is non-zero, keep that crop resident, with
+    // per crop row the number of observed pixels left of each column
+    std::vector<uint16_t> crop, rows;
+    qpso_host::build_crop(cl->depth, kCols, cl->rect, crop, rows);
     hipStream_t st = ctx->stream;
     int rc = QPSO_OK;
     if (hipMemcpyAsync(A + s->L.obs, crop.data(), crop.size() * 2, hipMemcpyHostToDevice, st) != hipSuccess ||
@@ -1221,8 +1098,8 @@ int qpso_server_submit(qpso_server* sv, uint32_t mesh_id, const qpso_cluster* cl
     r.job.cl = *cl;
     r.job.depth.assign(cl->depth, cl->depth + kFrame);
     r.bytes = qpso_solve_bytes(p, cl->rect);
-    const int S = rand_stride(p->n_iters);
-    r.table = qpso_sched::TableKey(rand_seed(p->seed), p->n_particles, S);
+    const int S = qpso_host::rand_stride(p->n_iters);
+    r.table = qpso_sched::TableKey(qpso_host::rand_seed(p->seed), p->n_particles, S);
     r.table_bytes = sizeof(float) * (uint64_t)S * (uint64_t)p->n_particles;
     r.particles = p->n_particles;
     r.iters = p->n_iters;

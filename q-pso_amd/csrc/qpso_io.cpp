@@ -5,6 +5,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <climits>
+#include <cstdint>
 #include <cstring>
 #include <fstream>
 #include <sstream>
@@ -116,7 +118,15 @@ int qpso_render_cpu(const float pose[7], const float* A, uint32_t n_faces, const
                     float* buf) {
     if (!pose || !A || !cam || !buf) return QPSO_E_INVAL;
     const size_t F = n_faces;
-    auto iround = [](float v) { return (int)(v + 0.5f); };
+    // (int)(v + 0.5f), truncating; out of range (undefined in C++) the
+    // reference's x86 host conversion gives INT32_MIN
+    auto iround = [](float v) {
+        const float t = v + 0.5f;
+        return (t > -2147483904.0f && t < 2147483648.0f) ? (int32_t)t : INT32_MIN;
+    };
+    // 32-bit integer arithmetic wraps, as on the reference's hardware
+    auto sub = [](int32_t a, int32_t b) { return (int32_t)((uint32_t)a - (uint32_t)b); };
+    auto mul = [](int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint32_t)b); };
     for (size_t i = 0; i < F; ++i) {
         float U[3], V[3], Z[3];
         for (int j = 0; j < 3; ++j) {
@@ -135,35 +145,38 @@ int qpso_render_cpu(const float pose[7], const float* A, uint32_t n_faces, const
         }
         const int X1 = iround(16.f * U[0]), X2 = iround(16.f * U[1]), X3 = iround(16.f * U[2]);
         const int Y1 = iround(16.f * V[0]), Y2 = iround(16.f * V[1]), Y3 = iround(16.f * V[2]);
-        const int DX12 = X1 - X2, DX23 = X2 - X3, DX31 = X3 - X1;
-        const int DY12 = Y1 - Y2, DY23 = Y2 - Y3, DY31 = Y3 - Y1;
-        const int tri = DX12 * DY31 - DX31 * DY12;
+        const int32_t DX12 = sub(X1, X2), DX23 = sub(X2, X3), DX31 = sub(X3, X1);
+        const int32_t DY12 = sub(Y1, Y2), DY23 = sub(Y2, Y3), DY31 = sub(Y3, Y1);
+        const int32_t tri = sub(mul(DX12, DY31), mul(DX31, DY12));
         if (tri <= 0) continue;
         const float inv = 1.0f / float(tri);
-        const int minx = (std::min(X1, std::min(X2, X3)) + 0xF) >> 4, maxx = (std::max(X1, std::max(X2, X3)) + 0xF) >> 4;
-        const int miny = (std::min(Y1, std::min(Y2, Y3)) + 0xF) >> 4, maxy = (std::max(Y1, std::max(Y2, Y3)) + 0xF) >> 4;
-        if (maxx - minx <= 0 || maxy - miny <= 0) continue;
-        int C2 = DY23 * X2 - DX23 * Y2 + (DY23 < 0 || (DY23 == 0 && DX23 > 0));
-        int C3 = DY31 * X3 - DX31 * Y3 + (DY31 < 0 || (DY31 == 0 && DX31 > 0));
-        int CY2 = C2 + DX23 * (miny << 4) - DY23 * (minx << 4);
-        int CY3 = C3 + DX31 * (miny << 4) - DY31 * (minx << 4);
+        auto bound = [](int32_t v) { return (int32_t)((uint32_t)v + 0xFu) >> 4; };
+        const int minx = bound(std::min(X1, std::min(X2, X3))), maxx = bound(std::max(X1, std::max(X2, X3)));
+        const int miny = bound(std::min(Y1, std::min(Y2, Y3))), maxy = bound(std::max(Y1, std::max(Y2, Y3)));
+        if ((int64_t)maxx - minx <= 0 || (int64_t)maxy - miny <= 0) continue;
+        // edge values (mod 2^32, as the reference's int arithmetic wraps on its
+        // hardware) evaluated directly at the first in-frame pixel: equal to
+        // the reference's walk from (minx, miny); off-frame pixels are clipped
+        const uint32_t C2 = (uint32_t)DY23 * (uint32_t)X2 - (uint32_t)DX23 * (uint32_t)Y2 +
+                            (DY23 < 0 || (DY23 == 0 && DX23 > 0));
+        const uint32_t C3 = (uint32_t)DY31 * (uint32_t)X3 - (uint32_t)DX31 * (uint32_t)Y3 +
+                            (DY31 < 0 || (DY31 == 0 && DX31 > 0));
+        const int x0 = std::max(minx, 0), x1 = std::min(maxx, kCols);
+        const int y0 = std::max(miny, 0), y1 = std::min(maxy, kRows);
         const float z2 = (Z[1] - Z[0]) * inv, z3 = (Z[2] - Z[0]) * inv;
-        for (int y = miny; y < maxy; ++y) {
-            int CX2 = CY2, CX3 = CY3;
-            for (int x = minx; x < maxx; ++x) {
-                const int cx2 = (CX2 + 0xF) >> 8, cx3 = (CX3 + 0xF) >> 8;
+        for (int y = y0; y < y1; ++y) {
+            uint32_t CX2 = C2 + (uint32_t)DX23 * ((uint32_t)y << 4) - (uint32_t)DY23 * ((uint32_t)x0 << 4);
+            uint32_t CX3 = C3 + (uint32_t)DX31 * ((uint32_t)y << 4) - (uint32_t)DY31 * ((uint32_t)x0 << 4);
+            for (int x = x0; x < x1; ++x) {
+                const int cx2 = (int32_t)(CX2 + 0xFu) >> 8, cx3 = (int32_t)(CX3 + 0xFu) >> 8;
                 float d = Z[0];
                 d += cx2 * z2;
                 d += cx3 * z3;
-                if (x >= 0 && x < kCols && y >= 0 && y < kRows) {  // frame clip
-                    float& b = buf[y * kCols + x];
-                    if (d < b) b = d;
-                }
-                CX2 -= DY23 << 4;
-                CX3 -= DY31 << 4;
+                float& b = buf[y * kCols + x];
+                if (d < b) b = d;
+                CX2 -= (uint32_t)DY23 << 4;
+                CX3 -= (uint32_t)DY31 << 4;
             }
-            CY2 += DX23 << 4;
-            CY3 += DX31 << 4;
         }
     }
     return QPSO_OK;

@@ -250,9 +250,10 @@ def test_big_mesh_without_runs(ctx):
 
 
 def test_packed_cache_far_vertices(ctx, scene1):
-    # the 201x133 cluster rect takes the packed vertex cache (16-bit X, Y);
+    # the 220x220 cluster rect takes the packed vertex cache (16-bit X, Y);
     # a pose close to the camera and off-axis puts vertices beyond 2047
-    # pixels, so that particle projects its faces where needed instead
+    # pixels (with the union box still under the guard's limit), so that
+    # particle projects its faces where needed instead
     scene, params = scene1
     poses = sample_poses(scene, params.n_particles, seed=31)
     rng = np.random.default_rng(31)
@@ -260,7 +261,7 @@ def test_packed_cache_far_vertices(ctx, scene1):
     for i in range(8):
         q = gt[3:] + rng.normal(0, 0.05, 4)
         q = q / np.linalg.norm(q)
-        poses[:, 5 * i] = [0.2 + 0.01 * i, rng.uniform(-0.02, 0.02), 0.15 + 0.005 * i, *q]
+        poses[:, 5 * i] = [0.2 + 0.01 * i, rng.uniform(-0.02, 0.02), 0.2 + 0.005 * i, *q]
     poses[:, 3] = [0.3, 0.0, 0.2, *gt[3:]]
     fit = check_eval(ctx, scene, params, poses, frames=False)
     far = [i for i in range(0, 40, 5)]
