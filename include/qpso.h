@@ -125,6 +125,21 @@ int qpso_mesh_release(qpso_ctx* ctx, uint32_t mesh_id);
 int qpso_estimate_pose(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl,
                        const qpso_params* p, qpso_result* out);
 
+/* qpso_estimate_pose without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc).  The solve's arena and random
+ * table live in a file-scope device array of QPSO_NOALLOC_BYTES that every
+ * GPU the library runs on holds until the process ends; a solve that needs
+ * more (about N <= 1024 at I <= 200 with a 640x480 rect fits) returns
+ * QPSO_E_NOMEM before touching the device.  It calls no hipMalloc, hipFree
+ * or graph routine; the mesh must already be uploaded.  Calls on one GPU
+ * share the array and run one at a time; it keeps the last call's
+ * contents, and each call reinitialises what it reads.  Results equal
+ * qpso_estimate_pose's. */
+#define QPSO_NOALLOC_BYTES (40u << 20)
+int qpso_estimate_pose_noalloc(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl,
+                               const qpso_params* p, qpso_result* out);
+
 /* The same block split in two so that the inputs are device-resident before
  * the solve runs: create uploads the cluster depth and the random table
  * (main.cu:1377-1557); run executes InitQPSO and the iteration loop

@@ -181,6 +181,9 @@ struct qpso_ctx {
         hipGraphExec_t exec = nullptr;
     } plan;
     int big_face = 128;  // default big-face threshold of new solves
+    // this device's copy of the allocation-free entry point's scratch
+    // (hipGetSymbolAddress at create; null if unavailable)
+    char* scratch = nullptr;
 };
 
 struct qpso_solve {
@@ -410,6 +413,10 @@ int qpso_create(int device, qpso_ctx** out) {
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->n_cu = prop.multiProcessorCount;
+    // with this device current: its copy of the no-allocation scratch
+    void* scratch = nullptr;
+    if (qpso_launch::scratch_address(device, &scratch) == hipSuccess) ctx->scratch = static_cast<char*>(scratch);
+    else hipGetLastError();  // leaves the entry point unavailable, nothing else
     if (const char* e = std::getenv("QPSO_RF_CFG")) {
         const int c = std::atoi(e);
         if (c >= 0 && c < qpso_launch::rf_num_configs()) ctx->rf_cfg = c;
@@ -665,6 +672,86 @@ int qpso_estimate_pose(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl, 
     r = qpso_solve_run(s, out);
     qpso_solve_destroy(s);
     return r;
+}
+
+// ---- the plain per-cluster call without device allocation -------------------
+namespace {
+// calls on one GPU share its scratch: they run one at a time
+std::mutex& scratch_mutex(int device) {
+    static std::mutex mu[64];
+    return mu[device & 63];
+}
+}  // namespace
+
+int qpso_estimate_pose_noalloc(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl, const qpso_params* p,
+                               qpso_result* out) {
+    if (!ctx || !out) return QPSO_E_INVAL;
+    if (int v = validate(cl, p)) return set_err(ctx, v, "invalid cluster or parameters");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (!ctx->scratch) return set_err(ctx, QPSO_E_HIP, "no-allocation scratch unavailable on this device");
+    auto mit = ctx->meshes.find(mesh_id);
+    if (mit == ctx->meshes.end() || mit->second.released) return set_err(ctx, QPSO_E_NOTFOUND, "unknown mesh id");
+    const int N = p->n_particles, I = p->n_iters, S = qpso_host::rand_stride(I);
+    const int rw = cl->rect[2] - cl->rect[0], rh = cl->rect[3] - cl->rect[1];
+    // the arena as a solve's, then its random table
+    const ArenaLayout L = layout(N, I, rw, rh, false);
+    const size_t rand_off = align_up(L.total, 256);
+    const size_t n_rand = (size_t)S * N;
+    if (rand_off + sizeof(float) * n_rand > QPSO_NOALLOC_SCRATCH_BYTES)
+        return set_err(ctx, QPSO_E_NOMEM,
+                       "solve needs " + std::to_string((rand_off + sizeof(float) * n_rand) >> 10) +
+                           " KiB, more than the no-allocation scratch (" +
+                           std::to_string(QPSO_NOALLOC_SCRATCH_BYTES >> 10) + " KiB)");
+    std::lock_guard<std::mutex> sg(scratch_mutex(ctx->device));
+    char* A = ctx->scratch;
+    float* rand = reinterpret_cast<float*>(A + rand_off);
+    QpsoSolveDev h;
+    fill_desc(h, mit->second, cl, p, A, L, rand, false, ctx->big_face);
+    QpsoSolveDev* d_desc = reinterpret_cast<QpsoSolveDev*>(A + L.desc);
+    hipStream_t st = ctx->stream;
+    // uploads from the caller's buffers and the stack: the descriptor, the
+    // cluster rect of the depth frame, the random table in chunks
+    HIP_TRY(ctx, hipMemcpyAsync(d_desc, &h, sizeof(h), hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemcpy2DAsync(A + L.obs, sizeof(uint16_t) * rw,
+                                  cl->depth + (size_t)cl->rect[1] * kCols + cl->rect[0], sizeof(uint16_t) * kCols,
+                                  sizeof(uint16_t) * rw, rh, hipMemcpyHostToDevice, st));
+    {
+        qpso_host::Mt19937 mt(qpso_host::rand_seed(p->seed));
+        float chunk[4096];
+        for (size_t k = 0; k < n_rand; k += 4096) {
+            const size_t n = std::min<size_t>(4096, n_rand - k);
+            for (size_t i = 0; i < n; ++i) chunk[i] = (float)((double)mt.next() / 4294967296.0);
+            HIP_TRY(ctx, hipMemcpyAsync(rand + k, chunk, sizeof(float) * n, hipMemcpyHostToDevice, st));
+            HIP_TRY(ctx, hipStreamSynchronize(st));  // the chunk is reused
+        }
+    }
+    HIP_TRY(ctx, qpso_launch::obs_rows(reinterpret_cast<const uint16_t*>(A + L.obs),
+                                       reinterpret_cast<uint16_t*>(A + L.obs_rows), rw, rh, st));
+    // InitQPSO clears the iteration, flags, counters and carried-pixel counts
+    HIP_TRY(ctx, qpso_launch::init(d_desc, 1, st));
+    const bool win = qpso_launch::rf_needs_windows(ctx->rf_cfg, h.V_eff);
+    for (int it = 0; it < I; ++it) {
+        HIP_TRY(ctx, qpso_launch::render_fitness(ctx->rf_cfg, win, d_desc, nullptr, N, st));
+        HIP_TRY(ctx, qpso_launch::pso_step(d_desc, 1, st));
+    }
+    float gb[8];
+    unsigned long long c[3];
+    int32_t flags = 0;
+    HIP_TRY(ctx, hipMemcpyAsync(gb, h.gbest, sizeof(gb), hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipMemcpyAsync(c, h.counters, sizeof(c), hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipMemcpyAsync(&flags, h.flags, sizeof(flags), hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    std::memcpy(out->pose, gb, sizeof(float) * 7);
+    out->fit = gb[7];
+    out->raster_px = c[0];
+    out->visit_px = c[1];
+    out->n_degenerate = (uint32_t)c[2];
+    out->status = (flags & QPSO_FLAG_LAUNCH) ? QPSO_E_HIP : (flags & QPSO_FLAG_STALE_OVERFLOW) ? QPSO_E_STALE : QPSO_OK;
+    out->latency_ms = 0.0;
+    if (out->status == QPSO_E_STALE) return set_err(ctx, QPSO_E_STALE, "stale-pixel list overflow");
+    if (out->status != QPSO_OK) return set_err(ctx, out->status, "render kernel launched without mesh windows");
+    return QPSO_OK;
 }
 
 int qpso_solve_get_trace(qpso_solve* s, qpso_trace* tr) {

@@ -5,6 +5,8 @@
 
 #include <stdint.h>
 
+#include "qpso.h"
+
 #define QPSO_EMPTY 60000          // maxVal_depthBufferInt (main.h:148)
 #define QPSO_AABB_MIN_SENT 50000  // max_AABB_ (main.h:150)
 #define QPSO_AABB_MAX_SENT (-1)   // min_AABB_ (main.h:151)
@@ -22,6 +24,9 @@
 // carried-over pixels kept per particle and iteration; more set
 // QPSO_FLAG_STALE_OVERFLOW
 #define QPSO_STALE_CAP 1024
+
+// device scratch of the allocation-free entry point (qpso_estimate_pose_noalloc)
+#define QPSO_NOALLOC_SCRATCH_BYTES QPSO_NOALLOC_BYTES
 
 struct QpsoStale {  // one carried-over depth-buffer pixel
     int32_t pos;    // (y << 16) | x

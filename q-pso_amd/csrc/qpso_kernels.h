@@ -31,6 +31,14 @@ hipError_t render_fitness(int cfg, int mode, const QpsoSolveDev* d_solves, const
 hipError_t pso_step(const QpsoSolveDev* d_solves, int n_solves, hipStream_t st, hipEvent_t ev0 = nullptr,
                     hipEvent_t ev1 = nullptr);
 
+// The address of `device`'s copy of the allocation-free entry point's
+// scratch (QPSO_NOALLOC_SCRATCH_BYTES); `device` must be current at the first
+// call for it, which takes the address once.
+hipError_t scratch_address(int device, void** ptr);
+// Per-row observed-pixel prefix counts of a device crop (rw x rh uint16 ->
+// rh x (rw + 1) uint16).
+hipError_t obs_rows(const uint16_t* d_crop, uint16_t* d_rows, int rw, int rh, hipStream_t st);
+
 }  // namespace qpso_launch
 
 #endif

@@ -18,6 +18,8 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "qpso_internal.h"
 #include "qpso_kernels.h"
 #include "qpso_math.h"
@@ -1238,6 +1240,53 @@ namespace qpso_launch {
 struct RfCfg {
     int bt, cap;
 };
+
+// ------------------------------------------------------------------------
+// scratch of the allocation-free entry point: a solve's arena and random
+// table, one copy per GPU, held until the process ends.  Host code never
+// passes `scratch` itself (in host code it names a stand-in, not device
+// memory); it uses the address hipGetSymbolAddress gives for the current
+// device.
+// ------------------------------------------------------------------------
+static __device__ __align__(256) char scratch[QPSO_NOALLOC_SCRATCH_BYTES];
+
+// The address is taken once per device (with that device current) and kept.
+hipError_t scratch_address(int device, void** ptr) {
+    static std::mutex mu;
+    static void* addr[64] = {};
+    if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(mu);
+    if (!addr[device]) {
+        void* a = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[device] = a;
+    }
+    *ptr = addr[device];
+    return hipSuccess;
+}
+
+// Per crop row, the number of observed (non-zero) pixels left of each column
+// ((rw + 1) counts a row), one thread per row.
+__global__ __launch_bounds__(256) void k_obs_rows(const uint16_t* __restrict__ crop, uint16_t* __restrict__ rows,
+                                                  int rw, int rh) {
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= rh) return;
+    const uint16_t* c = crop + (size_t)y * rw;
+    uint16_t* r = rows + (size_t)y * (rw + 1);
+    uint16_t n = 0;
+    r[0] = 0;
+    for (int x = 0; x < rw; ++x) {
+        n = (uint16_t)(n + (c[x] != 0 ? 1 : 0));
+        r[x + 1] = n;
+    }
+}
+
+hipError_t obs_rows(const uint16_t* d_crop, uint16_t* d_rows, int rw, int rh, hipStream_t st) {
+    if (rw <= 0 || rh <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_obs_rows, dim3((rh + 255) / 256), dim3(256), 0, st, d_crop, d_rows, rw, rh);
+    return hipGetLastError();
+}
 
 static const RfCfg kRfCfgs[] = {
     {1024, 40448},  // 1 workgroup / CU, 158 KiB tile

@@ -22,7 +22,7 @@ E_CANCELLED, E_TIMEOUT = -8, -9
 # Symbols every header in include/ declares (checked by the CPU test suite).
 EXPORTS = [
     "qpso_params_default", "qpso_create", "qpso_destroy", "qpso_last_error", "qpso_status_str",
-    "qpso_mesh_upload", "qpso_mesh_release", "qpso_estimate_pose", "qpso_solve_create",
+    "qpso_mesh_upload", "qpso_mesh_release", "qpso_estimate_pose", "qpso_estimate_pose_noalloc", "qpso_solve_create",
     "qpso_solve_run", "qpso_solve_get_trace", "qpso_solve_destroy", "qpso_solve_run_timed",
     "qpso_solve_run_reference_launches",
     "qpso_solve_run_batch", "qpso_autotune", "qpso_set_tuning",
@@ -113,6 +113,7 @@ def lib():
             "qpso_mesh_upload": (I32, [P, P, U32, P]),
             "qpso_mesh_release": (I32, [P, U32]),
             "qpso_estimate_pose": (I32, [P, U32, P, P, P]),
+            "qpso_estimate_pose_noalloc": (I32, [P, U32, P, P, P]),
             "qpso_solve_create": (I32, [P, U32, P, P, C.c_int, P]),
             "qpso_solve_run": (I32, [P, P]),
             "qpso_solve_get_trace": (I32, [P, P]),
