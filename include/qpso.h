@@ -66,6 +66,17 @@ typedef struct {
     uint32_t seed;         /* mt19937 seed; 0 means GSL's default 4357 */
     qpso_camera cam;
     int32_t face_cap;      /* 0 = every face; 3072 = the reference's 6x512 grid */
+    int32_t offframe;      /* 0: depth-buffer accesses outside the 640x480 frame
+                              are dropped (writes) or read as empty; 1: the
+                              reference's own indexing, idx = y*640 + x in 32-bit
+                              arithmetic (main.h:1442, main.cu:674, 688), so a
+                              pixel past the left or right edge lands in the
+                              neighbouring row of the particle's buffer; indices
+                              outside that buffer (another particle's in the
+                              reference) are still dropped.  Mode 1 keeps a
+                              640x480 int32 buffer per particle in device memory
+                              and runs in plain solves only (not batched, served,
+                              autotuned or allocation-free) */
 } qpso_params;
 
 /* One segmented cluster (GraphCannySeg::SegResults, GraphCannySeg.h:594-618). */

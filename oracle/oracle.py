@@ -29,7 +29,7 @@ class Params(C.Structure):
         ("c1", C.c_float), ("c2", C.c_float),
         ("x_lo", C.c_float * 3), ("x_hi", C.c_float * 3),
         ("rect", C.c_int * 4), ("num_points", C.c_uint32), ("face_cap", C.c_int),
-        ("cam", Cam),
+        ("cam", Cam), ("wrap", C.c_int),
     ]
 
 
@@ -76,6 +76,9 @@ def load(path):
     L.or_render.argtypes = [P, C.c_int, C.c_int, P, P, P, P, P]
     L.or_fitness.argtypes = [P, P, C.c_uint32, P, P, P]
     L.or_fitness.restype = C.c_float
+    L.or_render_ex.argtypes = [P, C.c_int, C.c_int, P, P, P, C.c_int, P, P]
+    L.or_fitness_ex.argtypes = [P, P, C.c_uint32, C.c_int, P, P, P]
+    L.or_fitness_ex.restype = C.c_float
     L.or_argmin.argtypes = [P, C.c_int]
     L.or_argmin.restype = C.c_int
     L.or_init.argtypes = [P, P, P, P, P, P, P]
@@ -104,8 +107,9 @@ def cam_struct(cam):
     return Cam(*[float(x) for x in cam])
 
 
-def make_params(n, iters, w, c1, c2, x_lo, x_hi, rect, num_points, cam, face_cap=0):
+def make_params(n, iters, w, c1, c2, x_lo, x_hi, rect, num_points, cam, face_cap=0, wrap=0):
     p = Params()
+    p.wrap = int(wrap)
     p.n_particles, p.n_iters, p.w, p.c1, p.c2 = n, iters, w, c1, c2
     for i in range(3):
         p.x_lo[i], p.x_hi[i] = float(x_lo[i]), float(x_hi[i])
@@ -137,7 +141,7 @@ def project(pose, v, cam):
     return out
 
 
-def render(mesh, pose, cam, rect, frame=None, face_cap=0):
+def render(mesh, pose, cam, rect, frame=None, face_cap=0, wrap=0):
     """Render one particle into `frame` (int32 480x640, min-accumulated)."""
     mesh = np.ascontiguousarray(mesh, np.float32)
     F = mesh.size // 9
@@ -147,15 +151,15 @@ def render(mesh, pose, cam, rect, frame=None, face_cap=0):
     r = (C.c_int * 4)(*[int(x) for x in rect])
     c = cam_struct(cam)
     info = RenderInfo()
-    lib().or_render(_p(mesh), F, face_cap, _p(pose), C.byref(c), r, _p(frame), C.byref(info))
+    lib().or_render_ex(_p(mesh), F, face_cap, _p(pose), C.byref(c), r, int(wrap), _p(frame), C.byref(info))
     return frame, info
 
 
-def fitness(rinfo, rect, num_points, frame, obs):
+def fitness(rinfo, rect, num_points, frame, obs, wrap=0):
     r = (C.c_int * 4)(*[int(x) for x in rect])
     obs = np.ascontiguousarray(obs, np.uint16)
     fi = FitnessInfo()
-    f = lib().or_fitness(C.byref(rinfo), r, int(num_points), _p(frame), _p(obs), C.byref(fi))
+    f = lib().or_fitness_ex(C.byref(rinfo), r, int(num_points), int(wrap), _p(frame), _p(obs), C.byref(fi))
     return np.float32(f), fi
 
 

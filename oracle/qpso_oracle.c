@@ -376,6 +376,56 @@ static uint64_t face_raster(const or_face* fc, int32_t* frame)
     return n;
 }
 
+/* The same in the reference-faithful off-frame mode: the whole bbox is
+ * walked as main.h:1424-1488 does and each pixel goes to the linear index
+ * idx = y*cols_ + x (main.h:1442), in wrapping 32-bit arithmetic; an index in
+ * [0, 640*480) is the particle's own buffer, any other one belongs to another
+ * particle or lies outside the allocation and is dropped. */
+static uint64_t face_raster_wrap(const or_face* fc, int32_t* frame)
+{
+    const int32_t X1 = fc->X[0], X2 = fc->X[1], X3 = fc->X[2];
+    const int32_t Y1 = fc->Y[0], Y2 = fc->Y[1], Y3 = fc->Y[2];
+    const int32_t DX12 = wsub(X1, X2), DX23 = wsub(X2, X3), DX31 = wsub(X3, X1);
+    const int32_t DY12 = wsub(Y1, Y2), DY23 = wsub(Y2, Y3), DY31 = wsub(Y3, Y1);
+    const int32_t tri = wsub(wmul(DX12, DY31), wmul(DX31, DY12));
+    if (tri <= 0) return 0;
+    if (fc->maxx - fc->minx <= 0 || fc->maxy - fc->miny <= 0) return 0;
+    const float inv = 1.0f * or_rcp((float)tri);
+    const int32_t FDX23 = wshl(DX23, 4), FDX31 = wshl(DX31, 4);
+    const int32_t FDY23 = wshl(DY23, 4), FDY31 = wshl(DY31, 4);
+    int32_t C2 = wsub(wmul(DY23, X2), wmul(DX23, Y2));
+    int32_t C3 = wsub(wmul(DY31, X3), wmul(DX31, Y3));
+    C2 = wadd(C2, (DY23 < 0 || (DY23 == 0 && DX23 > 0)) ? 1 : 0);
+    C3 = wadd(C3, (DY31 < 0 || (DY31 == 0 && DX31 > 0)) ? 1 : 0);
+    const int32_t MINY = wshl(fc->miny, 4), MINX = wshl(fc->minx, 4);
+    int32_t CY2 = wsub(wadd(C2, wmul(DX23, MINY)), wmul(DY23, MINX));
+    int32_t CY3 = wsub(wadd(C3, wmul(DX31, MINY)), wmul(DY31, MINX));
+    const float z1 = fc->Z[0];
+    const float z2 = (fc->Z[1] - fc->Z[0]) * inv;
+    const float z3 = (fc->Z[2] - fc->Z[0]) * inv;
+    uint64_t n = 0;
+    for (int32_t y = fc->miny; y < fc->maxy; ++y) {
+        int32_t CX2 = CY2, CX3 = CY3;
+        for (int32_t x = fc->minx; x < fc->maxx; ++x) {
+            const int32_t idx = wadd(wmul(y, OR_COLS), x);
+            if (idx >= 0 && idx < OR_FRAME) {
+                const int32_t cx2 = wadd(CX2, 15) >> 8;
+                const int32_t cx3 = wadd(CX3, 15) >> 8;
+                float d = fmaf((float)cx2, z2, z1);
+                d = fmaf((float)cx3, z3, d);
+                const int32_t di = or_rint_sat(d);
+                if (di < frame[idx]) frame[idx] = di;
+                ++n;
+            }
+            CX2 = wsub(CX2, FDY23);
+            CX3 = wsub(CX3, FDY31);
+        }
+        CY2 = wadd(CY2, FDX23);
+        CY3 = wadd(CY3, FDX31);
+    }
+    return n;
+}
+
 static int union_degenerate(const int32_t aabb[4], const int rect[4])
 {
     const int32_t minX = imin(aabb[0], rect[0]), maxX = imax(aabb[1], rect[2]);
@@ -387,6 +437,12 @@ static int union_degenerate(const int32_t aabb[4], const int rect[4])
 
 void or_render(const float* mesh, int F, int face_cap, const float pose[7],
                const or_cam* cam, const int rect[4], int32_t* frame, or_render_info* info)
+{
+    or_render_ex(mesh, F, face_cap, pose, cam, rect, 0, frame, info);
+}
+
+void or_render_ex(const float* mesh, int F, int face_cap, const float pose[7], const or_cam* cam,
+                  const int rect[4], int wrap, int32_t* frame, or_render_info* info)
 {
     const int Fe = (face_cap > 0 && face_cap < F) ? face_cap : F;
     /* sentinels of the AABB partial buffers (main.cu:1517-1535) take part in the
@@ -407,7 +463,8 @@ void or_render(const float* mesh, int F, int face_cap, const float pose[7],
     info->degenerate = degen;
     info->raster_px = 0;
     if (!degen)
-        for (int f = 0; f < Fe; ++f) info->raster_px += face_raster(&faces[f], frame);
+        for (int f = 0; f < Fe; ++f)
+            info->raster_px += wrap ? face_raster_wrap(&faces[f], frame) : face_raster(&faces[f], frame);
     free(faces);
 }
 
@@ -416,6 +473,12 @@ void or_render(const float* mesh, int F, int face_cap, const float pose[7],
 /* ------------------------------------------------------------------------ */
 float or_fitness(const or_render_info* rinfo, const int rect[4], uint32_t num_points,
                  int32_t* frame, const uint16_t* obs, or_fitness_info* finfo)
+{
+    return or_fitness_ex(rinfo, rect, num_points, 0, frame, obs, finfo);
+}
+
+float or_fitness_ex(const or_render_info* rinfo, const int rect[4], uint32_t num_points, int wrap,
+                    int32_t* frame, const uint16_t* obs, or_fitness_info* finfo)
 {
     memset(finfo, 0, sizeof(*finfo));
     if (rinfo->degenerate) {
@@ -449,8 +512,17 @@ float or_fitness(const or_render_info* rinfo, const int rect[4], uint32_t num_po
         const int32_t yb = (int32_t)truncf((float)t * rW); /* __float2int_rz(t*rcp(W)) */
         const int32_t px = minX + xb, py = minY + yb;
         if (py > maxY) break; /* yb is non-decreasing in t */
-        if (px < 0 || px >= OR_COLS || py < 0 || py >= OR_ROWS) continue; /* Q4: empty */
-        const size_t idx = (size_t)py * OR_COLS + (size_t)px;
+        size_t idx;
+        if (wrap) {
+            /* pxIdxRendered = pxY*cols_ + pxX (+ the particle's offset) and
+             * pxIdxKinect = pxY*cols_ + pxX (main.cu:674, 688), 32-bit */
+            const int32_t L = wadd(wmul(py, OR_COLS), px);
+            if (L < 0 || L >= OR_FRAME) continue; /* another buffer: read as empty */
+            idx = (size_t)L;
+        } else {
+            if (px < 0 || px >= OR_COLS || py < 0 || py >= OR_ROWS) continue; /* Q4: empty */
+            idx = (size_t)py * OR_COLS + (size_t)px;
+        }
         const int32_t zr = frame[idx];
         const int32_t zk = (int32_t)obs[idx];
         const int s = t & 2047;
@@ -649,8 +721,8 @@ int or_solve(const float* mesh, int F, const uint16_t* obs, const or_params* pr,
             or_render_info ri;
             or_fitness_info fi;
             int32_t* fr = frames + (size_t)p * OR_FRAME;
-            or_render(mesh, F, pr->face_cap, pose, &pr->cam, pr->rect, fr, &ri);
-            fit[p] = or_fitness(&ri, pr->rect, pr->num_points, fr, obs, &fi);
+            or_render_ex(mesh, F, pr->face_cap, pose, &pr->cam, pr->rect, pr->wrap, fr, &ri);
+            fit[p] = or_fitness_ex(&ri, pr->rect, pr->num_points, pr->wrap, fr, obs, &fi);
             rpx[p] += ri.raster_px;
             vpx[p] += fi.visit_px;
             degen[p] += ri.degenerate;

@@ -33,7 +33,12 @@
  * reads/writes out of bounds or races in these cases):
  *   Q4  depth-buffer accesses outside the 640x480 frame are dropped (writes)
  *       or read as empty (reads) instead of wrapping into other rows or
- *       other particles' buffers (main.h:1442, main.cu:674,688).
+ *       other particles' buffers (main.h:1442, main.cu:674,688).  With
+ *       `wrap` set (the reference-faithful mode) the linear index
+ *       y*640 + x, in wrapping 32-bit arithmetic, addresses the particle's
+ *       own buffer as the reference's does wherever it falls inside it;
+ *       only indices outside it (other particles' buffers or outside the
+ *       allocation in the reference) are dropped or read as empty.
  *   Q5  all faces are rendered (the reference ignores faces >= 3072,
  *       main.h:162-163); face_cap > 0 restores the cap.
  *   Q6  the random table has S = max(512, 12*(I+1)) entries per particle,
@@ -80,6 +85,7 @@ typedef struct {
     uint32_t num_points;         /* SegResults::num_points (main.cu:1318) */
     int face_cap;                /* 0 = all faces; 3072 = reference grid cap */
     or_cam cam;
+    int wrap;                    /* 0: off-frame accesses clipped (Q4); 1: linear-index wrap */
 } or_params;
 
 typedef struct {
@@ -112,6 +118,11 @@ void or_render(const float* mesh, int F, int face_cap, const float pose[7],
                const or_cam* cam, const int rect[4], int32_t* frame, or_render_info* info);
 float or_fitness(const or_render_info* rinfo, const int rect[4], uint32_t num_points,
                  int32_t* frame, const uint16_t* obs, or_fitness_info* finfo);
+/* the same with the off-frame mode chosen (wrap: see Q4) */
+void or_render_ex(const float* mesh, int F, int face_cap, const float pose[7], const or_cam* cam,
+                  const int rect[4], int wrap, int32_t* frame, or_render_info* info);
+float or_fitness_ex(const or_render_info* rinfo, const int rect[4], uint32_t num_points, int wrap,
+                    int32_t* frame, const uint16_t* obs, or_fitness_info* finfo);
 int or_argmin(const float* fit, int n);
 void or_init(const or_params* p, const float* rand, float* pos, float* vel,
              float* pbest, float* pbest_fit, float* gbest);

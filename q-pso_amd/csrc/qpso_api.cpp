@@ -74,10 +74,11 @@ void free_mesh(Mesh& m) {
 
 struct ArenaLayout {
     size_t desc, obs, obs_rows, pos, vel, pbest, pbest_fit, fit, gbest, iter, flags, counters;
-    size_t stale0, stale1, cnt0, cnt1, fit_tr, gbest_tr, argmin_tr, total;
+    size_t stale0, stale1, cnt0, cnt1, fit_tr, gbest_tr, argmin_tr, frames, total;
 };
 
-ArenaLayout layout(int N, int I, int rw, int rh, bool trace) {
+// `wrap`: the reference-faithful off-frame mode's full per-particle buffers
+ArenaLayout layout(int N, int I, int rw, int rh, bool trace, bool wrap = false) {
     ArenaLayout L{};
     size_t o = 0;
     auto take = [&](size_t bytes) { size_t r = o; o = align_up(o + bytes, 256); return r; };
@@ -102,6 +103,7 @@ ArenaLayout layout(int N, int I, int rw, int rh, bool trace) {
         L.gbest_tr = take(sizeof(float) * 8 * (size_t)I);
         L.argmin_tr = take(sizeof(int32_t) * (size_t)I);
     }
+    if (wrap) L.frames = take(sizeof(int32_t) * (size_t)N * kFrame);
     L.total = o;
     return L;
 }
@@ -154,6 +156,10 @@ void fill_desc(QpsoSolveDev& h, const Mesh& m, const qpso_cluster* cl, const qps
         h.fit_trace = reinterpret_cast<float*>(A + L.fit_tr);
         h.gbest_trace = reinterpret_cast<float*>(A + L.gbest_tr);
         h.argmin_trace = reinterpret_cast<int32_t*>(A + L.argmin_tr);
+    }
+    if (p->offframe) {
+        h.wrap = 1;
+        h.frames = reinterpret_cast<int32_t*>(A + L.frames);
     }
 }
 
@@ -225,6 +231,7 @@ int validate(const qpso_cluster* cl, const qpso_params* p) {
         return QPSO_E_INVAL;
     if (cl->num_points == 0) return QPSO_E_INVAL;
     if (!(p->t_bound >= 0.0f) || !(p->t_bound_z >= 0.0f)) return QPSO_E_INVAL;
+    if (p->offframe != 0 && p->offframe != 1) return QPSO_E_INVAL;
     return QPSO_OK;
 }
 
@@ -321,6 +328,20 @@ int read_result(qpso_solve* s, qpso_result* out, hipStream_t st) {
 }
 
 bool needs_windows(const qpso_solve* s, int cfg) { return qpso_launch::rf_needs_windows(cfg, s->h.V_eff); }
+
+// One render+fitness launch of a solve's particles (all, or the items given):
+// the reference-faithful off-frame mode has its own kernel.
+hipError_t launch_rf(const qpso_solve* s, int cfg, const QpsoSolveDev* d_desc, const QpsoItem* d_items, int n,
+                     hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
+    if (s->h.wrap) return qpso_launch::render_fitness_wrap(d_desc, d_items, n, st, ev0, ev1);
+    return qpso_launch::render_fitness(cfg, needs_windows(s, cfg), d_desc, d_items, n, st, ev0, ev1);
+}
+
+bool any_wrap(qpso_solve* const* solves, int n) {
+    for (int i = 0; i < n; ++i)
+        if (solves[i]->h.wrap) return true;
+    return false;
+}
 bool any_needs_windows(qpso_solve* const* solves, int n, int cfg) {
     for (int i = 0; i < n; ++i)
         if (needs_windows(solves[i], cfg)) return true;
@@ -330,9 +351,10 @@ bool any_needs_windows(qpso_solve* const* solves, int n, int cfg) {
 // Launch the whole iteration loop of one solve (plain form).
 int launch_solve(qpso_solve* s, hipStream_t st) {
     qpso_ctx* ctx = s->ctx;
+    if (s->h.wrap) HIP_TRY(ctx, qpso_launch::fill_frames(s->h.frames, (size_t)s->h.N * kFrame, st));
     HIP_TRY(ctx, qpso_launch::init(s->d_desc, 1, st));
     for (int it = 0; it < s->h.I; ++it) {
-        HIP_TRY(ctx, qpso_launch::render_fitness(s->rf_cfg, needs_windows(s, s->rf_cfg), s->d_desc, nullptr, s->h.N, st));
+        HIP_TRY(ctx, launch_rf(s, s->rf_cfg, s->d_desc, nullptr, s->h.N, st));
         HIP_TRY(ctx, qpso_launch::pso_step(s->d_desc, 1, st));
     }
     return QPSO_OK;
@@ -491,7 +513,7 @@ int qpso_mesh_release(qpso_ctx* ctx, uint32_t mesh_id) {
 uint64_t qpso_solve_bytes(const qpso_params* p, const int32_t rect[4]) {
     if (!p || !rect) return 0;
     const int rw = std::max(0, rect[2] - rect[0]), rh = std::max(0, rect[3] - rect[1]);
-    return layout(p->n_particles, p->n_iters, rw, rh, false).total;
+    return layout(p->n_particles, p->n_iters, rw, rh, false, p->offframe != 0).total;
 }
 
 int qpso_solve_create(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl, const qpso_params* p,
@@ -512,7 +534,7 @@ int qpso_solve_create(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl, c
     s->params = *p;
     s->mesh_id = mesh_id;
     std::memcpy(s->rect, cl->rect, sizeof(s->rect));
-    s->L = layout(N, I, rw, rh, s->trace);
+    s->L = layout(N, I, rw, rh, s->trace, p->offframe != 0);
     if (hipMalloc(&s->arena, s->L.total) != hipSuccess)
         return set_err(ctx, QPSO_E_NOMEM, "solve arena allocation failed");
     float* rand = nullptr;
@@ -601,14 +623,14 @@ int qpso_solve_run_reference_launches(qpso_solve* s, qpso_result* out) {
         e = hipStreamCreateWithFlags(&ps[k], hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&join[k], hipEventDisableTiming);
     }
-    const bool win = needs_windows(s, s->rf_cfg);
     float best = 0.0f;
+    if (e == hipSuccess && s->h.wrap) e = qpso_launch::fill_frames(s->h.frames, (size_t)N * kFrame, st);
     if (e == hipSuccess) e = qpso_launch::init(s->d_desc, 1, st);
     for (int it = 0; it < I && e == hipSuccess; ++it) {
         e = hipEventRecord(fork, st);
         for (int k = 0; k < kStreams && e == hipSuccess; ++k) e = hipStreamWaitEvent(ps[k], fork, 0);
         for (int p = 0; p < N && e == hipSuccess; ++p)
-            e = qpso_launch::render_fitness(s->rf_cfg, win, s->d_desc, d_items + p, 1, ps[p % kStreams]);
+            e = launch_rf(s, s->rf_cfg, s->d_desc, d_items + p, 1, ps[p % kStreams]);
         for (int k = 0; k < kStreams && e == hipSuccess; ++k) {
             e = hipEventRecord(join[k], ps[k]);
             if (e == hipSuccess) e = hipStreamWaitEvent(st, join[k], 0);
@@ -640,11 +662,11 @@ int qpso_solve_run_timed(qpso_solve* s, qpso_result* out, qpso_timing* tm) {
         for (auto& e : s->timed_ev) HIP_TRY(ctx, hipEventCreate(&e));
     }
     const std::vector<hipEvent_t>& ev = s->timed_ev;
-    const bool win = needs_windows(s, s->rf_cfg);
-    hipError_t e = qpso_launch::init(s->d_desc, 1, st, ev[0], ev[1]);
+    hipError_t e = hipSuccess;
+    if (s->h.wrap) e = qpso_launch::fill_frames(s->h.frames, (size_t)s->h.N * kFrame, st);
+    if (e == hipSuccess) e = qpso_launch::init(s->d_desc, 1, st, ev[0], ev[1]);
     for (int it = 0; it < I && e == hipSuccess; ++it) {
-        e = qpso_launch::render_fitness(s->rf_cfg, win, s->d_desc, nullptr, s->h.N, st, ev[2 + 4 * it],
-                                        ev[3 + 4 * it]);
+        e = launch_rf(s, s->rf_cfg, s->d_desc, nullptr, s->h.N, st, ev[2 + 4 * it], ev[3 + 4 * it]);
         if (e == hipSuccess) e = qpso_launch::pso_step(s->d_desc, 1, st, ev[4 + 4 * it], ev[5 + 4 * it]);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -687,6 +709,7 @@ int qpso_estimate_pose_noalloc(qpso_ctx* ctx, uint32_t mesh_id, const qpso_clust
                                qpso_result* out) {
     if (!ctx || !out) return QPSO_E_INVAL;
     if (int v = validate(cl, p)) return set_err(ctx, v, "invalid cluster or parameters");
+    if (p->offframe) return set_err(ctx, QPSO_E_INVAL, "the off-frame mode needs its own buffers: not allocation-free");
     std::lock_guard<std::mutex> g(ctx->mu);
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     if (!ctx->scratch) return set_err(ctx, QPSO_E_HIP, "no-allocation scratch unavailable on this device");
@@ -771,7 +794,9 @@ int qpso_solve_get_trace(qpso_solve* s, qpso_trace* tr) {
     if (tr->vel) HIP_TRY(ctx, hipMemcpy(tr->vel, A + s->L.vel, sizeof(float) * 7 * N, hipMemcpyDeviceToHost));
     if (tr->pbest) HIP_TRY(ctx, hipMemcpy(tr->pbest, A + s->L.pbest, sizeof(float) * 7 * N, hipMemcpyDeviceToHost));
     if (tr->pbest_fit) HIP_TRY(ctx, hipMemcpy(tr->pbest_fit, A + s->L.pbest_fit, sizeof(float) * N, hipMemcpyDeviceToHost));
-    if (tr->frames) {
+    if (tr->frames && s->h.wrap) {  // the buffers themselves
+        HIP_TRY(ctx, hipMemcpy(tr->frames, s->h.frames, sizeof(int32_t) * (size_t)N * kFrame, hipMemcpyDeviceToHost));
+    } else if (tr->frames) {
         // rebuild the reference's per-particle buffers after the last fitness pass:
         // empty except for the carried (never visited) pixels
         int32_t it = 0;
@@ -828,9 +853,13 @@ int qpso_solve_eval_poses(qpso_solve* s, int n, const float* poses, float* fit, 
     tmp.dbg_aabb = d_aabb;
     if (hipMalloc(&d_tmp, sizeof(QpsoSolveDev)) != hipSuccess) { cleanup(); return QPSO_E_NOMEM; }
     hipError_t e = hipMemcpyAsync(d_tmp, &tmp, sizeof(tmp), hipMemcpyHostToDevice, st);
-    // frames come from their own instantiation (it handles every mesh)
-    if (e == hipSuccess)
+    if (s->h.wrap) {  // the particles' own buffers start empty too
+        if (e == hipSuccess) e = qpso_launch::fill_frames(s->h.frames, (size_t)n * kFrame, st);
+        if (e == hipSuccess) e = qpso_launch::render_fitness_wrap(d_tmp, nullptr, n, st);
+    } else if (e == hipSuccess) {
+        // frames come from their own instantiation (it handles every mesh)
         e = qpso_launch::render_fitness(s->rf_cfg, frames ? 2 : needs_windows(s, s->rf_cfg) ? 1 : 0, d_tmp, nullptr, n, st);
+    }
     if (e == hipSuccess && fit) e = hipMemcpyAsync(fit, A + s->L.fit, sizeof(float) * n, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess && aabb) e = hipMemcpyAsync(aabb, d_aabb, sizeof(int32_t) * 4 * n, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess && frames)
@@ -847,6 +876,7 @@ int qpso_autotune(qpso_ctx* ctx, int n, qpso_solve* const* solves, int iters, qp
     if (!ctx || n <= 0 || !solves || iters <= 0 || (cap > 0 && !table)) return QPSO_E_INVAL;
     for (int i = 0; i < n; ++i)
         if (!solves[i] || solves[i]->ctx != ctx) return QPSO_E_INVAL;
+    if (any_wrap(solves, n)) return set_err(ctx, QPSO_E_INVAL, "the reference-faithful off-frame mode has one kernel: nothing to tune");
     static const int kBigFaces[] = {48, 96, 128, 160};
     const int n_cfg = qpso_launch::rf_num_configs();
     std::lock_guard<std::mutex> g(ctx->mu);
@@ -944,6 +974,7 @@ int qpso_solve_run_batch(qpso_ctx* ctx, int n, qpso_solve* const* solves, qpso_r
     if (!ctx || n <= 0 || !solves) return QPSO_E_INVAL;
     for (int i = 0; i < n; ++i)
         if (!solves[i] || solves[i]->ctx != ctx) return QPSO_E_INVAL;
+    if (any_wrap(solves, n)) return set_err(ctx, QPSO_E_INVAL, "the reference-faithful off-frame mode runs in plain solves only");
     std::lock_guard<std::mutex> g(ctx->mu);
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
@@ -1179,6 +1210,7 @@ int qpso_server_submit(qpso_server* sv, uint32_t mesh_id, const qpso_cluster* cl
                        uint64_t* ticket) {
     if (!sv || !ticket) return QPSO_E_INVAL;
     if (int v = validate(cl, p)) return v;
+    if (p->offframe) return QPSO_E_INVAL;  // the off-frame mode runs in plain solves only
     qpso_sched::Server<HipDev>::Request r;
     r.job.mesh_id = mesh_id;
     r.job.p = *p;

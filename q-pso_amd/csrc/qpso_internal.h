@@ -90,6 +90,10 @@ struct QpsoSolveDev {
     float* fit_trace;       // [I][N]
     float* gbest_trace;     // [I][8]
     int32_t* argmin_trace;  // [I]
+    // reference-faithful off-frame mode (qpso_params.offframe = 1): the
+    // reference's full per-particle depth buffers, [N][480][640], else null
+    int32_t wrap;
+    int32_t* frames;
     // debug outputs of the stage entry point (nullable)
     int32_t* dbg_frames;    // [N][480][640]
     int32_t* dbg_aabb;      // [N][4]

@@ -27,6 +27,12 @@ bool rf_needs_windows(int cfg, int v_eff);
 // beyond the cache); 2: any mesh with the frame dump of the stage call.
 hipError_t render_fitness(int cfg, int mode, const QpsoSolveDev* d_solves, const QpsoItem* d_items, int n_items,
                           hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+// The reference-faithful off-frame mode's render+fitness pass (the solves'
+// full per-particle buffers in device memory, linear-index wrap).
+hipError_t render_fitness_wrap(const QpsoSolveDev* d_solves, const QpsoItem* d_items, int n_items, hipStream_t st,
+                               hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+// Sets n_words depth words to the empty value 60000 (a kernel, not a memset).
+hipError_t fill_frames(int32_t* d_frames, size_t n_words, hipStream_t st);
 // argmin + pbest/gbest + particle update, one workgroup per solve.
 hipError_t pso_step(const QpsoSolveDev* d_solves, int n_solves, hipStream_t st, hipEvent_t ev0 = nullptr,
                     hipEvent_t ev1 = nullptr);

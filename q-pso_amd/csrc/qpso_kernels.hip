@@ -18,6 +18,7 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <mutex>
 
 #include "qpso_internal.h"
@@ -1047,6 +1048,256 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
 }
 
 // ------------------------------------------------------------------------
+// reference-faithful off-frame mode (qpso_params.offframe = 1).  The
+// reference's own per-particle int32 buffer of the whole frame lives in
+// device memory and is addressed as the reference does: a bbox pixel (x, y)
+// goes to idx = y*640 + x in wrapping 32-bit arithmetic (main.h:1442), so a
+// pixel past the left or right edge lands in the neighbouring row, and the
+// fitness scan reads and clears pxY*640 + pxX the same way (main.cu:674,
+// 688).  An index outside [0, 640*480) belongs to another particle's buffer
+// in the reference (or lies outside the allocation): dropped / read as empty.
+// Pixels the scan does not visit keep their depth in the buffer.  One face
+// per thread, global atomicMin, the scan over the 2048 slots: a slow path for
+// exact emulation.
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_fill_frames(int32_t* __restrict__ frames, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        frames[i] = QPSO_EMPTY;
+}
+
+// Observed depth at frame index L in [0, 640*480): the cluster crop inside
+// the rect, nothing elsewhere (the cluster depth is zero outside the cluster).
+__device__ __forceinline__ int32_t observed_at(const QpsoSolveDev& S, int32_t L) {
+    const int32_t y = L / kCols, x = L - y * kCols;
+    if (x < S.rect[0] || x >= S.rect[2] || y < S.rect[1] || y >= S.rect[3]) return 0;
+    const int32_t rw = S.rect[2] - S.rect[0];
+    return (int32_t)((const QG uint16_t*)S.obs)[(y - S.rect[1]) * rw + (x - S.rect[0])];
+}
+
+template <int BT>
+__global__ __launch_bounds__(BT) void k_render_fitness_wrap(const QpsoSolveDev* __restrict__ solves,
+                                                           const QpsoItem* __restrict__ items) {
+    constexpr int SPT = 2048 / BT;  // fitness slots per thread: tid + j BT
+    constexpr int32_t kFrame = kRows * kCols;
+    __shared__ int32_t s_box[4];
+    __shared__ int32_t s_degen;
+    __shared__ uint32_t s_cnt[3];
+    __shared__ unsigned long long s_px[2];
+    __shared__ float red[BT];
+    const int tid = threadIdx.x;
+    const QpsoItem item = items ? items[blockIdx.x] : QpsoItem{0, (int32_t)blockIdx.x};
+    const QpsoSolveDev& S = solves[item.solve];
+    const int p = item.particle;
+    const int iter = __builtin_amdgcn_readfirstlane(((const QG int32_t*)S.iter)[0]);
+    if (iter >= S.I) return;  // this solve has finished
+    if (!S.frames) {  // a solve without the buffers (the host never launches this)
+        if (tid == 0) {
+            S.fit[p] = __builtin_nanf("");
+            atomicOr(S.flags, QPSO_FLAG_LAUNCH);
+        }
+        return;
+    }
+    const int N = S.N;
+    float pose[7];
+#pragma unroll
+    for (int d = 0; d < 7; ++d) pose[d] = ((const QG float*)S.pos)[d * N + p];
+    const Cam cam{S.fx, S.fy, S.cx, S.cy};
+    if (tid == 0) {
+        s_box[0] = QPSO_AABB_MIN_SENT; s_box[1] = QPSO_AABB_MAX_SENT;
+        s_box[2] = QPSO_AABB_MIN_SENT; s_box[3] = QPSO_AABB_MAX_SENT;
+        s_degen = 0;
+        s_cnt[0] = s_cnt[1] = s_cnt[2] = 0;
+        s_px[0] = s_px[1] = 0;
+    }
+    __syncthreads();
+    // AABB of every face (main.h:1190-1304, main.cu:423-546): faces
+    // [0, F_eff) use exactly the vertices [0, V_eff)
+    {
+        const QG float* __restrict__ wx = (const QG float*)S.verts;
+        const QG float* __restrict__ wy = wx + S.V;
+        const QG float* __restrict__ wz = wy + S.V;
+        int32_t xl = INT32_MAX, xh = INT32_MIN, yl = INT32_MAX, yh = INT32_MIN;
+        int dg = 0;
+        for (int v = tid; v < S.V_eff; v += BT) {
+            int32_t X, Y;
+            float Z;
+            dg |= vertex_fixed(pose, wx[v], wy[v], wz[v], cam, X, Y, Z) ? 1 : 0;
+            xl = min(xl, X); xh = max(xh, X);
+            yl = min(yl, Y); yh = max(yh, Y);
+        }
+        int bx0 = QPSO_AABB_MIN_SENT, bx1 = QPSO_AABB_MAX_SENT, by0 = QPSO_AABB_MIN_SENT, by1 = QPSO_AABB_MAX_SENT;
+        if (xl <= xh) {
+            bx0 = wadd(xl, 15) >> 4; bx1 = wadd(xh, 15) >> 4;
+            by0 = wadd(yl, 15) >> 4; by1 = wadd(yh, 15) >> 4;
+        }
+        bx0 = wave_min(bx0); bx1 = wave_max(bx1);
+        by0 = wave_min(by0); by1 = wave_max(by1);
+        dg = wave_max(dg);
+        if ((tid & 63) == 0) {
+            atomicMin(&s_box[0], bx0); atomicMax(&s_box[1], bx1);
+            atomicMin(&s_box[2], by0); atomicMax(&s_box[3], by1);
+            if (dg) s_degen = 1;
+        }
+    }
+    __syncthreads();
+    const int32_t ax0 = s_box[0], ax1 = s_box[1], ay0 = s_box[2], ay1 = s_box[3];
+    const int32_t minX = min(ax0, S.rect[0]), maxX = max(ax1, S.rect[2]);
+    const int32_t minY = min(ay0, S.rect[1]), maxY = max(ay1, S.rect[3]);
+    const long long W64 = (long long)maxX - minX, H64 = (long long)maxY - minY;
+    const bool degen = s_degen != 0 || W64 <= 0 || H64 <= 0 || W64 * H64 > (long long)QPSO_AREA_LIMIT;
+    if (S.dbg_aabb && tid == 0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) S.dbg_aabb[p * 4 + i] = s_box[i];
+    }
+    int32_t* __restrict__ buf = S.frames + (size_t)p * kFrame;
+    if (degen) {  // guard G: NaN fitness, buffer reset
+        for (int i = tid; i < kFrame; i += BT) buf[i] = QPSO_EMPTY;
+        if (S.dbg_frames)
+            for (int i = tid; i < kFrame; i += BT) S.dbg_frames[(size_t)p * kFrame + i] = QPSO_EMPTY;
+        if (tid == 0) {
+            S.fit[p] = __builtin_nanf("");
+            atomicAdd(&S.counters[2], 1ull);
+        }
+        return;
+    }
+
+    // raster: bbox fill (main.h:1355-1488), one face per thread, the whole bbox
+    unsigned long long px_r = 0;
+    for (int f = tid; f < S.F_eff; f += BT) {
+        Face fc;
+        face_project(S.corners, f, pose, cam, fc);
+        if (!face_front(fc)) continue;
+        Setup su;
+        face_setup(fc, su);
+        for (int32_t y = fc.miny; y < fc.maxy; ++y) {
+            // where y*640 + x cannot wrap (|y| <= 3e6, |x| <= 2^26) the
+            // columns that land in the buffer are one interval; elsewhere
+            // (union boxes of 2^23 pixels at most are then a few columns
+            // wide) every pixel is checked
+            int32_t xa = fc.minx, xb = fc.maxx;
+            if (y >= -3000000 && y <= 3000000) {
+                const int32_t base = y * kCols;
+                xa = max(xa, -base);
+                xb = min(xb, kFrame - base);
+                if (xa >= xb) continue;
+            }
+            int32_t CX2 = wsub(wadd(su.CY2, wmul(y - fc.miny, su.FDX23)), wmul(xa - fc.minx, su.FDY23));
+            int32_t CX3 = wsub(wadd(su.CY3, wmul(y - fc.miny, su.FDX31)), wmul(xa - fc.minx, su.FDY31));
+            const int32_t row = wmul(y, kCols);
+            for (int32_t x = xa; x < xb; ++x) {
+                const int32_t L = wadd(row, x);
+                if ((uint32_t)L < (uint32_t)kFrame) {
+                    atomicMin(buf + L, pixel_depth(CX2, CX3, su));
+                    ++px_r;
+                }
+                CX2 = wsub(CX2, su.FDY23);
+                CX3 = wsub(CX3, su.FDY31);
+            }
+        }
+    }
+    __syncthreads();
+    if (S.dbg_frames) {  // the rendered buffer, before the fitness pass
+        int32_t* fr = S.dbg_frames + (size_t)p * kFrame;
+        for (int i = tid; i < kFrame; i += BT) fr[i] = __hip_atomic_load(buf + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+
+    // fitness (main.cu:596-760): slot t & 2047 sums its pixels in increasing t
+    const int32_t W = (int32_t)W64, H = (int32_t)H64;
+    const int32_t T = ((W * H + 1023) >> 10) << 10;  // NumIters * 1024
+    const float rW = rcp((float)W);
+    float acc_e[SPT];
+    uint32_t n_seg = 0, n_tr = 0, n_rend = 0;
+    unsigned long long px_f = 0;
+    auto pixel = [&](int32_t t, int32_t& L) {  // false past the last row or outside the buffer
+        const int32_t yb = row_of(t, rW);
+        const int32_t xb = t % W;  // x_bar = tIdxLoop % W_bar
+        const int32_t py = minY + yb;
+        if (py > maxY) return 0;
+        L = wadd(wmul(py, kCols), minX + xb);
+        return (uint32_t)L < (uint32_t)kFrame ? 1 : 2;
+    };
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+        acc_e[j] = 0.0f;
+        for (int32_t t = tid + j * BT; t < T; t += 2048) {
+            int32_t L = 0;
+            const int r = pixel(t, L);
+            if (r == 0) break;  // the row index does not decrease with t
+            if (r == 2) continue;
+            ++px_f;
+            const int32_t zr = __hip_atomic_load(buf + L, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int32_t zk = observed_at(S, L);
+            if (zr != QPSO_EMPTY) {
+                ++n_rend;
+                if (zk == 0) {
+                    ++n_tr;
+                } else {
+                    const float e = (float)wsub(zr, zk) * 0.001f;
+                    acc_e[j] = __builtin_fmaf(e, e, acc_e[j]);
+                }
+            } else if (zk != 0) {
+                ++n_seg;
+            }
+        }
+    }
+    __syncthreads();
+    // visited pixels are cleared after every read (a pixel visited twice reads
+    // its rendered value both times); the others carry their depth
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+        for (int32_t t = tid + j * BT; t < T; t += 2048) {
+            int32_t L = 0;
+            const int r = pixel(t, L);
+            if (r == 0) break;
+            if (r == 1) buf[L] = QPSO_EMPTY;
+        }
+    }
+
+    // FitnessReductionBallot's halving tree (main.h:1522-1573)
+#pragma unroll
+    for (int stride = 1024; stride >= BT; stride >>= 1) {
+#pragma unroll
+        for (int j = 0; j < stride / BT; ++j) acc_e[j] = acc_e[j] + acc_e[j + stride / BT];
+    }
+    const uint32_t cs = wave_sum(n_seg), ct = wave_sum(n_tr), cr = wave_sum(n_rend);
+    px_r = wave_sum64(px_r);
+    px_f = wave_sum64(px_f);
+    if ((tid & 63) == 0) {
+        atomicAdd(&s_cnt[0], cs); atomicAdd(&s_cnt[1], ct); atomicAdd(&s_cnt[2], cr);
+        atomicAdd(&s_px[0], px_r);
+        atomicAdd(&s_px[1], px_f);
+    }
+    red[tid] = acc_e[0];
+    __syncthreads();
+    if (tid < 64) {
+        float r[BT / 64];
+#pragma unroll
+        for (int k = 0; k < BT / 64; ++k) r[k] = red[tid + 64 * k];
+#pragma unroll
+        for (int s2 = BT / 128; s2 >= 1; s2 >>= 1) {
+#pragma unroll
+            for (int k = 0; k < s2; ++k) r[k] = r[k] + r[k + s2];
+        }
+        float v = r[0];
+#pragma unroll
+        for (int stride = 32; stride >= 1; stride >>= 1) {
+            const float o = __shfl_down(v, stride);
+            v = v + o;
+        }
+        if (tid == 0) {
+            const uint32_t seg = s_cnt[0], tr = s_cnt[1], nr = s_cnt[2];
+            const float inv_r = rcp((float)nr);
+            const float ratio = (float)seg * rcp((float)S.num_points);
+            const float ratioT = (float)tr * inv_r;
+            const float zerr = v * inv_r;
+            S.fit[p] = (float)__builtin_fma((double)(ratio + ratioT), 0.05, (double)zerr);
+            atomicAdd(&S.counters[0], s_px[0]);
+            atomicAdd(&S.counters[1], s_px[1]);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
 // argmin + pbest/gbest + particle update, one workgroup per solve
 // ------------------------------------------------------------------------
 __device__ __forceinline__ bool better(float av, int ai, float bv, int bi) {
@@ -1341,6 +1592,24 @@ hipError_t render_fitness(int cfg, int mode, const QpsoSolveDev* d_solves, const
         case 4: launch_rf<256, 10112>(mode, d_solves, d_items, n_items, st, ev0, ev1); break;
         default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t render_fitness_wrap(const QpsoSolveDev* d_solves, const QpsoItem* d_items, int n_items, hipStream_t st,
+                               hipEvent_t ev0, hipEvent_t ev1) {
+    if (n_items <= 0) return hipSuccess;
+    if (ev0 || ev1)
+        hipExtLaunchKernelGGL(k_render_fitness_wrap<256>, dim3(n_items), dim3(256), 0, st, ev0, ev1, 0, d_solves,
+                              d_items);
+    else
+        hipLaunchKernelGGL(k_render_fitness_wrap<256>, dim3(n_items), dim3(256), 0, st, d_solves, d_items);
+    return hipGetLastError();
+}
+
+hipError_t fill_frames(int32_t* d_frames, size_t n_words, hipStream_t st) {
+    if (n_words == 0) return hipSuccess;
+    const size_t blocks = std::min<size_t>((n_words + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_frames, dim3((unsigned)blocks), dim3(256), 0, st, d_frames, n_words);
     return hipGetLastError();
 }
 

@@ -12,8 +12,10 @@
 // and the CPU renders of the first 10 particles' final poses, 0.csv .. 9.csv.
 // Like the reference's 6x512-thread render grid (main.h:162-163) it renders
 // faces 0..3071 only; QPSO_FACE_CAP overrides that (0: every face).
+// QPSO_OFFFRAME=1 selects the reference's own off-frame indexing (pixels past
+// the left or right edge land in the neighbouring row; qpso_params.offframe).
 // Optional environment: QPSO_PARTICLES, QPSO_ITERS, QPSO_FX/FY/CX/CY,
-// QPSO_FACE_CAP.
+// QPSO_FACE_CAP, QPSO_OFFFRAME.
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -63,6 +65,7 @@ int main(int argc, char** argv) {
     p.cam.cx = env_d("QPSO_CX", p.cam.cx);
     p.cam.cy = env_d("QPSO_CY", p.cam.cy);
     p.face_cap = (int)env_d("QPSO_FACE_CAP", 3072);  // fxptBlocks * fxptThsPerBlock
+    p.offframe = (int)env_d("QPSO_OFFFRAME", 0);     // 1: the reference's off-frame indexing
 
     std::vector<uint16_t> cdepth(QPSO_ROWS * QPSO_COLS);
     qpso_cluster cl{};

@@ -43,7 +43,7 @@ class Params(C.Structure):
     _fields_ = [
         ("n_particles", C.c_int32), ("n_iters", C.c_int32), ("w", C.c_double),
         ("c1", C.c_float), ("c2", C.c_float), ("t_bound", C.c_float), ("t_bound_z", C.c_float),
-        ("seed", C.c_uint32), ("cam", Camera), ("face_cap", C.c_int32),
+        ("seed", C.c_uint32), ("cam", Camera), ("face_cap", C.c_int32), ("offframe", C.c_int32),
     ]
 
 
@@ -178,8 +178,9 @@ def default_params():
 
 
 def make_params(n_particles, n_iters, w=0.3, c1=1.0, c2=1.0, cam=(525.0, 525.0, 319.5, 239.5),
-                t_bound=0.3, t_bound_z=0.3, seed=0, face_cap=0):
+                t_bound=0.3, t_bound_z=0.3, seed=0, face_cap=0, offframe=0):
     p = default_params()
+    p.offframe = offframe
     p.n_particles, p.n_iters, p.w, p.c1, p.c2 = n_particles, n_iters, w, c1, c2
     p.t_bound, p.t_bound_z, p.seed, p.face_cap = t_bound, t_bound_z, seed, face_cap
     p.cam = Camera(*[float(x) for x in cam])

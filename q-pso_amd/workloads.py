@@ -181,6 +181,20 @@ def config4_mix(seed=0, n_stress=64, n_small=64, stress_iters=200):
     return big, small, jobs
 
 
+def edge_config(seed=0, n_particles=64, n_iters=12):
+    """The 1,280-face object cut by the left frame edge (its centre about 25 px
+    from it), for the off-frame modes: many particles' boxes cross x = 0."""
+    rng = np.random.default_rng(5000 + seed)
+    mesh = qpso.mesh_icosphere(3, 50.0, 35.0, 25.0)
+    z = rng.uniform(650.0, 750.0)
+    pose = np.array([(25.0 - CAM[2]) * z / CAM[0], rng.uniform(-40.0, 40.0), z, *qpso.random_quat(5000 + seed)],
+                    np.float32)
+    depth, labels = qpso.scene_render([(mesh, pose)], CAM, 1100.0, 2.0, 0.05, seed + 5)
+    scene = Scene([(mesh, pose)], depth, labels)
+    params = qpso.make_params(n_particles, n_iters, w=0.7, c1=1.5, c2=1.5, cam=CAM, seed=0, offframe=1)
+    return scene, params
+
+
 # FX, FY, CX, CY of the reference (main.h:30-33)
 ACCV_CAM = (572.41140, 573.57043, 325.26110, 242.04899)
 
@@ -209,4 +223,4 @@ def oracle_params(oracle_mod, scene, params, i=0):
     hi = [cen[0] + tb, cen[1] + tb, cen[2] + tbz]
     cam = (params.cam.fx, params.cam.fy, params.cam.cx, params.cam.cy)
     return oracle_mod.make_params(params.n_particles, params.n_iters, params.w, params.c1, params.c2,
-                                  lo, hi, c["rect"], c["num_points"], cam, params.face_cap)
+                                  lo, hi, c["rect"], c["num_points"], cam, params.face_cap, params.offframe)

@@ -62,6 +62,10 @@ def test_solve_bytes_grows_with_swarm():
     p.n_particles = 256
     b = qpso.lib().qpso_solve_bytes(C.byref(p), (C.c_int32 * 4)(0, 0, 100, 100))
     assert 0 < a < b
+    # the reference-faithful off-frame mode adds a 640x480 int32 buffer per particle
+    p.offframe = 1
+    c = qpso.lib().qpso_solve_bytes(C.byref(p), (C.c_int32 * 4)(0, 0, 100, 100))
+    assert b + 256 * 640 * 480 * 4 <= c < b + 256 * 640 * 480 * 4 + 4096
 
 
 def soa_faces(soa):

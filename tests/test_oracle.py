@@ -155,9 +155,11 @@ def one_face_mesh(v0, v1, v2):
     return np.concatenate([V[:, 0], V[:, 1], V[:, 2]]).astype(np.float32)
 
 
-def reference_raster_literal(uvz, frame):
+def reference_raster_literal(uvz, frame, wrap=False):
     """main.h:1326-1488 written as the reference's literal incremental walk over
-    the whole bbox (no clipping of the walk), writes kept when in frame."""
+    the whole bbox (no clipping of the walk), writes kept when in frame; with
+    `wrap`, written to idx = y*cols_ + x (main.h:1442, 32-bit) when that index
+    lies in the particle's own buffer."""
     X = [rint_sat_ref(np.float32(16.0) * np.float32(a[0])) for a in uvz]
     Y = [rint_sat_ref(np.float32(16.0) * np.float32(a[1])) for a in uvz]
     Z = [np.float32(a[2]) for a in uvz]
@@ -184,7 +186,12 @@ def reference_raster_literal(uvz, frame):
             cx3 = wrap32(CX3 + 15) >> 8
             d = fma32(np.float32(cx3), z3, fma32(np.float32(cx2), z2, Z[0]))
             di = rint_sat_ref(d)
-            if 0 <= x < 640 and 0 <= y < 480:
+            if wrap:
+                idx = wrap32(y * 640 + x)
+                if 0 <= idx < 640 * 480:
+                    frame.flat[idx] = min(frame.flat[idx], di)
+                    n += 1
+            elif 0 <= x < 640 and 0 <= y < 480:
                 frame[y, x] = min(frame[y, x], di)
                 n += 1
             CX2, CX3 = wrap32(CX2 - FDY23), wrap32(CX3 - FDY31)
@@ -211,6 +218,62 @@ def test_raster_matches_literal_reference_walk():
         assert info.raster_px == n
         done += n > 0
     assert done > 100
+
+
+def test_raster_wrap_matches_literal_reference_walk():
+    # the reference-faithful off-frame mode: triangles across the left and the
+    # right frame edges land, past the edge, in the neighbouring row
+    rng = np.random.default_rng(12)
+    rect = [0, 200, 40, 240]
+    ident = np.array([0, 0, 0, 1, 0, 0, 0], np.float32)
+    wrapped = 0
+    for trial in range(300):
+        z = rng.uniform(0.6, 1.0)
+        edge_x = rng.choice([-319.5, 320.5]) / 525.0 * z  # x of the left or right frame edge at depth z
+        base = np.array([edge_x, rng.uniform(-0.05, 0.05), z])
+        vs = [base + rng.normal(0, 0.006, 3) for _ in range(3)]
+        mesh = one_face_mesh(*vs)
+        frame, info = O.render(mesh, ident, CAM, rect, wrap=1)
+        uvz = [O.project(ident, np.array(v, np.float32), CAM) for v in vs]
+        ref = np.full((480, 640), 60000, np.int32)
+        n = reference_raster_literal(uvz, ref, wrap=True)
+        assert np.array_equal(frame, ref), trial
+        assert info.raster_px == n
+        clip, _ = O.render(mesh, ident, CAM, rect)
+        wrapped += not np.array_equal(clip, frame)
+    assert wrapped > 50
+
+
+@pytest.mark.parametrize("W,H,left", [(41, 3, 7), (64, 20, 13), (121, 2, 5)])
+def test_fitness_wrap_matches_thread_level_restatement(W, H, left):
+    # a union box that starts `left` columns left of the frame: those pixels
+    # are read (and cleared) at the end of the row above
+    rng = np.random.default_rng(W * 7 + H)
+    rect = [0, 100, W - left, 100 + H]
+    aabb = [-left, W - left - 1, 100, 100 + H]
+    frame = np.full((480, 640), 60000, np.int32)
+    obs = np.zeros((480, 640), np.uint16)
+    for arr, lo, frac in ((frame, 700, 0.7), (obs, 700, 0.8)):
+        m = rng.random((480, 640)) < frac
+        sel = np.zeros((480, 640), bool)
+        sel[99:102 + H, :] = True  # the rows the box touches, both edges
+        arr[m & sel] = rng.integers(lo, lo + 60, (m & sel).sum())
+    info = O.RenderInfo()
+    for i in range(4):
+        info.aabb[i] = aabb[i]
+    ref_fit, reads, nR = reference_fitness_threads(aabb, rect, W * H, frame, obs, wrap=True)
+    fr = frame.copy()
+    fit, fi = O.fitness(info, rect, W * H, fr, obs, wrap=1)
+    assert fi.n_rendered == nR
+    assert np.float32(fit).view(np.uint32) == np.float32(ref_fit).view(np.uint32)
+    visited = np.zeros((480, 640), bool)
+    for (py, px) in reads:
+        visited[py, px] = True
+    assert visited[100:100 + H, 640 - left:].any()  # reads wrapped to the right edge, one row up
+    assert np.array_equal(fr[visited], np.full(visited.sum(), 60000))
+    assert np.array_equal(fr[~visited], frame[~visited])
+    clip_fit, _ = O.fitness(info, rect, W * H, frame.copy(), obs)
+    assert np.float32(clip_fit).view(np.uint32) != np.float32(fit).view(np.uint32)
 
 
 def test_raster_is_bbox_fill_and_flat_depth():
@@ -268,8 +331,10 @@ def test_row_index_census_matches_survey():
     assert {41, 61, 109, 115, 121, 122, 123, 218, 230, 244} <= set(bad)
 
 
-def reference_fitness_threads(aabb, rect, num_points, frame, obs):
-    """main.cu:590-760 with 1024 explicit threads and 2048 shared slots."""
+def reference_fitness_threads(aabb, rect, num_points, frame, obs, wrap=False):
+    """main.cu:590-760 with 1024 explicit threads and 2048 shared slots; with
+    `wrap`, pixels are read at pxY*cols_ + pxX (main.cu:674, 688, 32-bit)
+    when that index lies in the buffer."""
     minX, maxX = min(aabb[0], rect[0]), max(aabb[1], rect[2])
     minY, maxY = min(aabb[2], rect[1]), max(aabb[3], rect[3])
     W, H = maxX - minX, maxY - minY
@@ -290,7 +355,12 @@ def reference_fitness_threads(aabb, rect, num_points, frame, obs):
             px, py = minX + xb, minY + yb
             if not (py <= maxY):
                 continue
-            if not (0 <= px < 640 and 0 <= py < 480):
+            if wrap:
+                L = wrap32(py * 640 + px)
+                if not (0 <= L < 640 * 480):
+                    continue
+                py, px = divmod(L, 640)
+            elif not (0 <= px < 640 and 0 <= py < 480):
                 continue
             zr, zk = int(snap[py, px]), int(obs[py, px])
             reads[(py, px)] = reads.get((py, px), 0) + 1
