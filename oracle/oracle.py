@@ -13,7 +13,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+# QPSO_ORACLE_LIB selects another build of the oracle (the sanitizer build
+# of tests/test_sanitizers.py)
+LIB_PATH = os.environ.get("QPSO_ORACLE_LIB") or os.path.join(HERE, "_build", "liboracle.so")
 
 ROWS, COLS = 480, 640
 EMPTY = 60000
