@@ -514,6 +514,38 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             c = wrap ? w : c;
         }
     };
+    // per-pixel depth over [x0, x1) x [y0, y1) by K lanes together: lane j
+    // starts at pixel j of the rectangle in row-major order and every step
+    // moves K pixels on (sy = K / w rows and sx = K % w columns, one wrap at
+    // most); the minima do not depend on which lane writes which pixel
+    auto raster_lanes = [&](const Face& fc, const Setup& su, int32_t x0, int32_t x1, int32_t y0, int32_t y1, int32_t j,
+                            int32_t K) {
+        if (x0 >= x1 || y0 >= y1) return;
+        const int32_t w = x1 - x0, n = __mul24(w, y1 - y0);
+        const int32_t sy = K / w, sx = K - __mul24(sy, w);
+        int32_t dy = j / w, dx = j - __mul24(dy, w);
+        int32_t CX2 = wsub(wadd(su.CY2, wmul(y0 + dy - fc.miny, su.FDX23)), wmul(x0 + dx - fc.minx, su.FDY23));
+        int32_t CX3 = wsub(wadd(su.CY3, wmul(y0 + dy - fc.miny, su.FDX31)), wmul(x0 + dx - fc.minx, su.FDY31));
+        const int32_t step2 = wsub(wmul(sy, su.FDX23), wmul(sx, su.FDY23));
+        const int32_t step3 = wsub(wmul(sy, su.FDX31), wmul(sx, su.FDY31));
+        const int32_t fix2 = wadd(su.FDX23, wmul(w, su.FDY23));  // x -= w, y += 1
+        const int32_t fix3 = wadd(su.FDX31, wmul(w, su.FDY31));
+        const int32_t astep = __mul24(sy, DW) + sx, afix = DW - w;
+        int32_t off = __mul24(y0 + dy - db0, DW) + (x0 + dx - dx0);
+        for (int32_t m = j; m < n; m += K) {
+            atomicMin(tile + off, pixel_depth(CX2, CX3, su));
+            dx += sx;
+            CX2 = wadd(CX2, step2);
+            CX3 = wadd(CX3, step3);
+            off += astep;
+            if (dx >= w) {
+                dx -= w;
+                CX2 = wadd(CX2, fix2);
+                CX3 = wadd(CX3, fix3);
+                off += afix;
+            }
+        }
+    };
     // a queued big face walked by the 64 lanes of a wave: lane l starts at
     // pixel l of the rectangle in row-major order and every step moves 64
     // pixels on, i.e. sy = 64 / w rows and sx = 64 % w columns, with one wrap
@@ -814,14 +846,21 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         }
         if (lane < cnt) draw(ring[(head + lane) & (kRing - 1)]);
         __syncthreads();
-        // queued small faces: per-pixel depth over their D part, one face per thread
+        // queued small faces: per-pixel depth over their D part, K lanes per
+        // face (a power of two, as many as the threads allow, at most 64),
+        // lane j of a face taking its pixels j, j + K, j + 2K, ...
         const int nq = min(s_nq, kQCap);
-        for (int q = tid; q < nq; q += BT) {
-            Face fc;
-            Setup su;
-            draw_face(queue[q], fc);
-            face_setup(fc, su);
-            raster_exact(fc, su, max(fc.minx, dx0), min(fc.maxx, dx1), max(fc.miny, db0), min(fc.maxy, db1));
+        if (nq > 0) {
+            const int share = BT / nq;
+            const int K = share >= 64 ? 64 : share >= 1 ? 1 << (31 - __builtin_clz(share)) : 1;
+            const int j = tid & (K - 1);
+            for (int q = tid / K; q < nq; q += BT / K) {
+                Face fc;
+                Setup su;
+                draw_face(queue[q], fc);
+                face_setup(fc, su);
+                raster_lanes(fc, su, max(fc.minx, dx0), min(fc.maxx, dx1), max(fc.miny, db0), min(fc.maxy, db1), j, K);
+            }
         }
         // queued big faces: one face per wave, taken in turn
         const int nqb = min(s_nqb, kBQCap);
@@ -907,6 +946,12 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             // first sum (a pixel outside Q reads as empty and observed 0,
             // which adds nothing), then each slot sums in increasing t
             constexpr int U = SPT >= 8 ? 1 : 8 / SPT;
+            // Q in union-box offsets (x, y), and the tile and crop indices as
+            // y * pitch + x + constant
+            const int32_t qxa = qx0 - minX, qxb = qx1 - minX, qya = qy0 - minY, qyb = qy1 - minY;
+            // (minY - db0 can reach -2^23 - 480, beyond __mul24: wrapping 32-bit
+            // products, exact modulo 2^32, as is each index they add up to)
+            const int32_t tc = (minY - db0) * DW + (minX - dx0), oc = (minY - ry0) * rw + (minX - rx0);
             for (int32_t base = (tlo / 2048) * 2048; base < thi; base += 2048 * U) {
                 int32_t zr[U][SPT], zk[U][SPT];
 #pragma unroll
@@ -920,10 +965,9 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
                         const int32_t y = row_of(t, vis.rW);
                         int32_t x = t - __mul24(y, W);
                         if (x >= W) x -= W;  // t = kW mapped to row k-1
-                        const int32_t px = minX + x, py = minY + y;
-                        if (px < qx0 || px >= qx1 || py < qy0 || py >= qy1) continue;
-                        zr[u][j] = tile[__mul24(py - db0, DW) + px - dx0];
-                        zk[u][j] = (int32_t)obs[__mul24(py - ry0, rw) + (px - rx0)];
+                        if (x < qxa || x >= qxb || y < qya || y >= qyb) continue;
+                        zr[u][j] = tile[__mul24(y, DW) + x + tc];
+                        zk[u][j] = (int32_t)obs[__mul24(y, rw) + x + oc];
                     }
                 }
 #pragma unroll
