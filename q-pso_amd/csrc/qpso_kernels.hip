@@ -952,41 +952,77 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             // (minY - db0 can reach -2^23 - 480, beyond __mul24: wrapping 32-bit
             // products, exact modulo 2^32, as is each index they add up to)
             const int32_t tc = (minY - db0) * DW + (minX - dx0), oc = (minY - ry0) * rw + (minX - rx0);
+            // Thread tid walks t = t0 + m BT (m = 0, 1, ...; slot j = m mod SPT)
+            // and keeps the true quotient and remainder y, x of t by W from
+            // step to step (BT = sq W + sr).  The fp32 row index differs from
+            // the true one only at t = yW (x = 0): for t below 2^23 the
+            // product t rcp(W) carries a relative error under 2^-23, less
+            // than x / t for 0 < x < W.  There it is y - 1 when row y is
+            // "short", and t visits (0, y - 1); column 0 of the union box lies
+            // in Q only when qxa = 0.  A t inside Q in (x, y) is at least tlo
+            // and below k1 W + 1, so of [tlo, thi) only t < T is left to test.
+            const uint32_t xn = (uint32_t)(qxb - qxa), yn = (uint32_t)(qyb - qya);
+            const int32_t sq = BT / W, sr = BT - sq * W;
+            const bool col0 = qxa == 0;
+            // the crop addressed by a 32-bit byte offset from its base
+            const QG char* __restrict__ ob = (const QG char*)obs;
+            int32_t t = (tlo / 2048) * 2048 + tid;
+            int32_t y = row_of(t, vis.rW), x = t - y * W;
+            if (x >= W) { x -= W; ++y; }
+            // the three counters per wave, from ballots (scalar adds)
+            uint32_t w_rend = 0, w_tr = 0, w_seg = 0;
             for (int32_t base = (tlo / 2048) * 2048; base < thi; base += 2048 * U) {
                 int32_t zr[U][SPT], zk[U][SPT];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
 #pragma unroll
                     for (int j = 0; j < SPT; ++j) {
-                        const int32_t t = base + u * 2048 + j * BT + tid;  // slot t & 2047 = j*BT + tid
+                        // t = base + u 2048 + j BT + tid, slot j BT + tid; for
+                        // t < thi <= T <= 2^23, y < 2^23 fits the 24-bit products
+                        int32_t yv = y;
+                        if (col0) {
+                            // lanes hold consecutive t: a wave meets x = 0 once
+                            // per W values of t, so the fp32 row is formed only then
+                            const bool z = x == 0;
+                            if (__ballot(z) != 0ull) {
+                                // kept as a branch (not evaluated for every t)
+                                __asm__ volatile("" ::: "memory");
+                                if (z && row_of(t, vis.rW) != y) yv = y - 1;
+                            }
+                        }
                         zr[u][j] = QPSO_EMPTY;
                         zk[u][j] = 0;
-                        if (t < tlo || t >= thi) continue;
-                        const int32_t y = row_of(t, vis.rW);
-                        int32_t x = t - __mul24(y, W);
-                        if (x >= W) x -= W;  // t = kW mapped to row k-1
-                        if (x < qxa || x >= qxb || y < qya || y >= qyb) continue;
-                        zr[u][j] = tile[__mul24(y, DW) + x + tc];
-                        zk[u][j] = (int32_t)obs[__mul24(y, rw) + x + oc];
+                        if ((uint32_t)t < (uint32_t)thi && (uint32_t)(x - qxa) < xn && (uint32_t)(yv - qya) < yn) {
+                            zr[u][j] = tile[__mul24(yv, DW) + x + tc];
+                            const uint32_t oi = (uint32_t)(__mul24(yv, rw) + x + oc);  // >= 0: a rect pixel
+                            zk[u][j] = (int32_t)*(const QG uint16_t*)(ob + 2u * oi);
+                        }
+                        t += BT;
+                        x += sr;
+                        y += sq;
+                        if (x >= W) { x -= W; ++y; }
                     }
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
 #pragma unroll
                     for (int j = 0; j < SPT; ++j) {
-                        if (zr[u][j] != QPSO_EMPTY) {
-                            ++n_rend;
-                            if (zk[u][j] == 0) {
-                                ++n_tr;
-                            } else {
-                                const float e = (float)wsub(zr[u][j], zk[u][j]) * 0.001f;
-                                acc_e[j] = __builtin_fmaf(e, e, acc_e[j]);
-                            }
-                        } else if (zk[u][j] != 0) {
-                            ++n_seg;
-                        }
+                        // an empty or unobserved pixel adds fma(0, 0, acc) = acc
+                        // (acc >= +0)
+                        const bool r = zr[u][j] != QPSO_EMPTY, o = zk[u][j] != 0;
+                        const uint64_t mr = __ballot(r), mo = __ballot(o);
+                        w_rend += (uint32_t)__popcll(mr);
+                        w_tr += (uint32_t)__popcll(mr & ~mo);
+                        w_seg += (uint32_t)__popcll(mo & ~mr);
+                        const float e = (float)((r && o) ? wsub(zr[u][j], zk[u][j]) : 0) * 0.001f;
+                        acc_e[j] = __builtin_fmaf(e, e, acc_e[j]);
                     }
                 }
+            }
+            if ((tid & 63) == 0) {
+                n_rend += w_rend;
+                n_tr += w_tr;
+                n_seg += w_seg;
             }
         }
 

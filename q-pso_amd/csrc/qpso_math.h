@@ -372,7 +372,7 @@ QD int32_t depth_at(const Face& fc, const Setup& s, int32_t x, int32_t y) {
 }
 
 // Fitness row index y(t) = trunc(f32(t) * rcp(f32(W))) (main.cu:645).
-QD int32_t row_of(int32_t t, float rW) { return (int32_t)__builtin_truncf((float)t * rW); }
+QD int32_t row_of(int32_t t, float rW) { return (int32_t)((float)t * rW); }  // the conversion truncates
 
 #undef QD
 }  // namespace qpso
