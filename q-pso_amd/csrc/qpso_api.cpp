@@ -190,6 +190,11 @@ struct qpso_ctx {
     // this device's copy of the allocation-free entry point's scratch
     // (hipGetSymbolAddress at create; null if unavailable)
     char* scratch = nullptr;
+    // solve arenas come from this stream-ordered pool where the device has
+    // one: a solve's memory is then released in the order of the stream that
+    // ran it, with no wait (hipFree waits for the whole device); freed
+    // memory stays in the pool for reuse until the context is destroyed
+    hipMemPool_t pool = nullptr;
 };
 
 struct qpso_solve {
@@ -237,6 +242,30 @@ int validate(const qpso_cluster* cl, const qpso_params* p) {
 
 // The random table of (seed, N, S), generated and uploaded on first use; the
 // caller holds a reference until release_rand.  Caller holds ctx->mu.
+// A blocking copy on the given stream.  The library never uses the null
+// stream: that would take one of the device's hardware queues (four per
+// process by default, handed to the least used first), which the serving
+// lanes need one each to run side by side.
+hipError_t copy_on(hipStream_t st, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, st);
+    return e == hipSuccess ? hipStreamSynchronize(st) : e;
+}
+
+// A solve arena: from the context's pool in the order of its stream (the
+// caller synchronises that stream before another one uses it), or from
+// hipMalloc.
+hipError_t alloc_arena(qpso_ctx* ctx, size_t bytes, char** p) {
+    if (ctx->pool) return hipMallocFromPoolAsync(reinterpret_cast<void**>(p), bytes, ctx->pool, ctx->stream);
+    return hipMalloc(reinterpret_cast<void**>(p), bytes);
+}
+
+// Release an arena after the work queued on `st` (pooled: without waiting;
+// otherwise hipFree, which waits for the whole device).
+void free_arena(qpso_ctx* ctx, char* p, hipStream_t st) {
+    if (ctx->pool) hipFreeAsync(p, st);
+    else hipFree(p);
+}
+
 int get_rand(qpso_ctx* ctx, const qpso_sched::TableKey& key, float** out) {
     auto it = ctx->rand_cache.find(key);
     if (it != ctx->rand_cache.end()) {
@@ -252,7 +281,7 @@ int get_rand(qpso_ctx* ctx, const qpso_sched::TableKey& key, float** out) {
     qpso_host::rand_table(seed, (size_t)S * N, h);
     float* d = nullptr;
     HIP_TRY(ctx, hipMalloc(&d, h.size() * sizeof(float)));
-    if (hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+    if (copy_on(ctx->stream, d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
         hipFree(d);
         return set_err(ctx, QPSO_E_HIP, "random table upload failed");
     }
@@ -295,27 +324,43 @@ void invalidate_plan(qpso_ctx* ctx) {
 
 // A finished solve's result, read on stream `st`; touches no context state
 // (the serving thread calls it without the context lock).
-int read_result_status(qpso_solve* s, qpso_result* out, hipStream_t st) {
+// A solve's result words as they lie in its arena: gbest pose and fitness,
+// the work counters, the error flags.
+struct ResultWords {
     float g[8];
     unsigned long long c[3];
-    int32_t flags = 0;
-    if (hipMemcpyAsync(g, s->arena + s->L.gbest, sizeof(g), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipMemcpyAsync(c, s->arena + s->L.counters, sizeof(c), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipMemcpyAsync(&flags, s->arena + s->L.flags, sizeof(flags), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-        return QPSO_E_HIP;
-    const int status = (flags & QPSO_FLAG_LAUNCH) ? QPSO_E_HIP
-                       : (flags & QPSO_FLAG_STALE_OVERFLOW) ? QPSO_E_STALE : QPSO_OK;
+    int32_t flags;
+    int32_t pad;
+};
+
+// Enqueue the copies of the result words into `w` on `st`.
+hipError_t copy_result(const qpso_solve* s, ResultWords* w, hipStream_t st) {
+    hipError_t e = hipMemcpyAsync(w->g, s->arena + s->L.gbest, sizeof(w->g), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(w->c, s->arena + s->L.counters, sizeof(w->c), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(&w->flags, s->arena + s->L.flags, sizeof(w->flags), hipMemcpyDeviceToHost, st);
+    return e;
+}
+
+// The result from its words; returns its status.
+int result_from(const ResultWords& w, qpso_result* out) {
+    const int status = (w.flags & QPSO_FLAG_LAUNCH) ? QPSO_E_HIP
+                       : (w.flags & QPSO_FLAG_STALE_OVERFLOW) ? QPSO_E_STALE : QPSO_OK;
     if (out) {
-        std::memcpy(out->pose, g, sizeof(float) * 7);
-        out->fit = g[7];
-        out->raster_px = c[0];
-        out->visit_px = c[1];
-        out->n_degenerate = (uint32_t)c[2];
+        std::memcpy(out->pose, w.g, sizeof(float) * 7);
+        out->fit = w.g[7];
+        out->raster_px = w.c[0];
+        out->visit_px = w.c[1];
+        out->n_degenerate = (uint32_t)w.c[2];
         out->status = status;
         out->latency_ms = 0.0;
     }
     return status;
+}
+
+int read_result_status(qpso_solve* s, qpso_result* out, hipStream_t st) {
+    ResultWords w{};
+    if (copy_result(s, &w, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) return QPSO_E_HIP;
+    return result_from(w, out);
 }
 
 int read_result(qpso_solve* s, qpso_result* out, hipStream_t st) {
@@ -434,7 +479,23 @@ int qpso_create(int device, qpso_ctx** out) {
         return QPSO_E_HIP;
     }
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->n_cu = prop.multiProcessorCount;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        ctx->n_cu = prop.multiProcessorCount;
+        if (prop.memoryPoolsSupported) {
+            hipMemPoolProps pp{};
+            pp.allocType = hipMemAllocationTypePinned;
+            pp.handleTypes = hipMemHandleTypeNone;
+            pp.location.type = hipMemLocationTypeDevice;
+            pp.location.id = device;
+            hipMemPool_t pool = nullptr;
+            uint64_t keep = UINT64_MAX;
+            if (hipMemPoolCreate(&pool, &pp) == hipSuccess) {
+                if (hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess) ctx->pool = pool;
+                else hipMemPoolDestroy(pool);
+            }
+            hipGetLastError();  // without a pool the arenas come from hipMalloc
+        }
+    }
     // with this device current: its copy of the no-allocation scratch
     void* scratch = nullptr;
     if (qpso_launch::scratch_address(device, &scratch) == hipSuccess) ctx->scratch = static_cast<char*>(scratch);
@@ -454,6 +515,10 @@ int qpso_destroy(qpso_ctx* ctx) {
     invalidate_plan(ctx);
     for (auto& kv : ctx->meshes) free_mesh(kv.second);
     for (auto& kv : ctx->rand_cache) hipFree(kv.second.d);
+    if (ctx->pool) {
+        hipDeviceSynchronize();
+        hipMemPoolDestroy(ctx->pool);
+    }
     hipStreamDestroy(ctx->stream);
     delete ctx;
     return QPSO_OK;
@@ -479,13 +544,13 @@ int qpso_mesh_upload(qpso_ctx* ctx, const float* soa, uint32_t n_faces, uint32_t
     if (e == hipSuccess) e = hipMalloc(&m.verts, sizeof(float) * verts.size());
     if (e == hipSuccess) e = hipMalloc(&m.faces, sizeof(QpsoFaceRef) * order.size());
     if (e == hipSuccess) e = hipMalloc(&m.runs, sizeof(int32_t) * std::max<size_t>(runs.size(), 1));
-    if (e == hipSuccess) e = hipMemcpy(m.verts, verts.data(), sizeof(float) * verts.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = copy_on(ctx->stream, m.verts, verts.data(), sizeof(float) * verts.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess)
-        e = hipMemcpy(m.corners, corners.data(), sizeof(float) * corners.size(), hipMemcpyHostToDevice);
+        e = copy_on(ctx->stream, m.corners, corners.data(), sizeof(float) * corners.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess)
-        e = hipMemcpy(m.faces, order.data(), sizeof(QpsoFaceRef) * order.size(), hipMemcpyHostToDevice);
+        e = copy_on(ctx->stream, m.faces, order.data(), sizeof(QpsoFaceRef) * order.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !runs.empty())
-        e = hipMemcpy(m.runs, runs.data(), sizeof(int32_t) * runs.size(), hipMemcpyHostToDevice);
+        e = copy_on(ctx->stream, m.runs, runs.data(), sizeof(int32_t) * runs.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         free_mesh(m);
         return set_err(ctx, e == hipErrorOutOfMemory ? QPSO_E_NOMEM : QPSO_E_HIP,
@@ -535,11 +600,15 @@ int qpso_solve_create(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl, c
     s->mesh_id = mesh_id;
     std::memcpy(s->rect, cl->rect, sizeof(s->rect));
     s->L = layout(N, I, rw, rh, s->trace, p->offframe != 0);
-    if (hipMalloc(&s->arena, s->L.total) != hipSuccess)
+    if (alloc_arena(ctx, s->L.total, &s->arena) != hipSuccess)
         return set_err(ctx, QPSO_E_NOMEM, "solve arena allocation failed");
     float* rand = nullptr;
     s->rand_key = qpso_sched::TableKey(qpso_host::rand_seed(p->seed), N, S);
-    if (int r = get_rand(ctx, s->rand_key, &rand)) { hipFree(s->arena); return r; }
+    if (int r = get_rand(ctx, s->rand_key, &rand)) {
+        free_arena(ctx, s->arena, ctx->stream);
+        hipStreamSynchronize(ctx->stream);
+        return r;
+    }
 
     fill_desc(s->h, mit->second, cl, p, s->arena, s->L, rand, s->trace, ctx->big_face);
     s->rf_cfg = ctx->rf_cfg;
@@ -558,7 +627,8 @@ int qpso_solve_create(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl, c
         hipStreamSynchronize(st) != hipSuccess)
         rc = set_err(ctx, QPSO_E_HIP, "solve upload failed");
     if (rc) {
-        hipFree(s->arena);
+        free_arena(ctx, s->arena, st);
+        hipStreamSynchronize(st);
         release_rand(ctx, s->rand_key);
         return rc;
     }
@@ -567,17 +637,17 @@ int qpso_solve_create(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl, c
     return QPSO_OK;
 }
 
-int qpso_solve_destroy(qpso_solve* s) {
-    if (!s) return QPSO_E_INVAL;
+namespace {
+
+// Destroy a solve whose work all lies on stream `st`; its arena is released
+// in that stream's order.
+void solve_release(qpso_solve* s, hipStream_t st) {
     qpso_ctx* ctx = s->ctx;
-    std::lock_guard<std::mutex> g(ctx->mu);
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
     auto& P = ctx->plan;
     if (std::find(P.solves.begin(), P.solves.end(), s) != P.solves.end()) invalidate_plan(ctx);
     drop_solve_graph(s);
     for (auto& e : s->timed_ev) hipEventDestroy(e);
-    hipFree(s->arena);
+    free_arena(ctx, s->arena, st);
     release_rand(ctx, s->rand_key);
     auto mit = ctx->meshes.find(s->mesh_id);
     if (mit != ctx->meshes.end() && --mit->second.refs == 0 && mit->second.released) {
@@ -585,6 +655,17 @@ int qpso_solve_destroy(qpso_solve* s) {
         ctx->meshes.erase(mit);
     }
     delete s;
+}
+
+}  // namespace
+
+int qpso_solve_destroy(qpso_solve* s) {
+    if (!s) return QPSO_E_INVAL;
+    qpso_ctx* ctx = s->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    solve_release(s, ctx->stream);
     return QPSO_OK;
 }
 
@@ -617,7 +698,7 @@ int qpso_solve_run_reference_launches(qpso_solve* s, qpso_result* out) {
         if (d_items) hipFree(d_items);
     };
     hipError_t e = hipMalloc(&d_items, sizeof(QpsoItem) * N);
-    if (e == hipSuccess) e = hipMemcpy(d_items, items.data(), sizeof(QpsoItem) * N, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = copy_on(st, d_items, items.data(), sizeof(QpsoItem) * N, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
     for (int k = 0; k < kStreams && e == hipSuccess; ++k) {
         e = hipStreamCreateWithFlags(&ps[k], hipStreamNonBlocking);
@@ -787,25 +868,25 @@ int qpso_solve_get_trace(qpso_solve* s, qpso_trace* tr) {
     if ((tr->fit_trace || tr->gbest_trace || tr->argmin_trace) && !s->trace)
         return set_err(ctx, QPSO_E_INVAL, "solve was created without traces");
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    if (tr->fit_trace) HIP_TRY(ctx, hipMemcpy(tr->fit_trace, A + s->L.fit_tr, sizeof(float) * (size_t)I * N, hipMemcpyDeviceToHost));
-    if (tr->gbest_trace) HIP_TRY(ctx, hipMemcpy(tr->gbest_trace, A + s->L.gbest_tr, sizeof(float) * 8 * (size_t)I, hipMemcpyDeviceToHost));
-    if (tr->argmin_trace) HIP_TRY(ctx, hipMemcpy(tr->argmin_trace, A + s->L.argmin_tr, sizeof(int32_t) * (size_t)I, hipMemcpyDeviceToHost));
-    if (tr->pos) HIP_TRY(ctx, hipMemcpy(tr->pos, A + s->L.pos, sizeof(float) * 7 * N, hipMemcpyDeviceToHost));
-    if (tr->vel) HIP_TRY(ctx, hipMemcpy(tr->vel, A + s->L.vel, sizeof(float) * 7 * N, hipMemcpyDeviceToHost));
-    if (tr->pbest) HIP_TRY(ctx, hipMemcpy(tr->pbest, A + s->L.pbest, sizeof(float) * 7 * N, hipMemcpyDeviceToHost));
-    if (tr->pbest_fit) HIP_TRY(ctx, hipMemcpy(tr->pbest_fit, A + s->L.pbest_fit, sizeof(float) * N, hipMemcpyDeviceToHost));
+    if (tr->fit_trace) HIP_TRY(ctx, copy_on(ctx->stream, tr->fit_trace, A + s->L.fit_tr, sizeof(float) * (size_t)I * N, hipMemcpyDeviceToHost));
+    if (tr->gbest_trace) HIP_TRY(ctx, copy_on(ctx->stream, tr->gbest_trace, A + s->L.gbest_tr, sizeof(float) * 8 * (size_t)I, hipMemcpyDeviceToHost));
+    if (tr->argmin_trace) HIP_TRY(ctx, copy_on(ctx->stream, tr->argmin_trace, A + s->L.argmin_tr, sizeof(int32_t) * (size_t)I, hipMemcpyDeviceToHost));
+    if (tr->pos) HIP_TRY(ctx, copy_on(ctx->stream, tr->pos, A + s->L.pos, sizeof(float) * 7 * N, hipMemcpyDeviceToHost));
+    if (tr->vel) HIP_TRY(ctx, copy_on(ctx->stream, tr->vel, A + s->L.vel, sizeof(float) * 7 * N, hipMemcpyDeviceToHost));
+    if (tr->pbest) HIP_TRY(ctx, copy_on(ctx->stream, tr->pbest, A + s->L.pbest, sizeof(float) * 7 * N, hipMemcpyDeviceToHost));
+    if (tr->pbest_fit) HIP_TRY(ctx, copy_on(ctx->stream, tr->pbest_fit, A + s->L.pbest_fit, sizeof(float) * N, hipMemcpyDeviceToHost));
     if (tr->frames && s->h.wrap) {  // the buffers themselves
-        HIP_TRY(ctx, hipMemcpy(tr->frames, s->h.frames, sizeof(int32_t) * (size_t)N * kFrame, hipMemcpyDeviceToHost));
+        HIP_TRY(ctx, copy_on(ctx->stream, tr->frames, s->h.frames, sizeof(int32_t) * (size_t)N * kFrame, hipMemcpyDeviceToHost));
     } else if (tr->frames) {
         // rebuild the reference's per-particle buffers after the last fitness pass:
         // empty except for the carried (never visited) pixels
         int32_t it = 0;
-        HIP_TRY(ctx, hipMemcpy(&it, A + s->L.iter, sizeof(int32_t), hipMemcpyDeviceToHost));
+        HIP_TRY(ctx, copy_on(ctx->stream, &it, A + s->L.iter, sizeof(int32_t), hipMemcpyDeviceToHost));
         const int par = it & 1;
         std::vector<int32_t> cnt(N);
         std::vector<QpsoStale> ent((size_t)N * kStaleCap);
-        HIP_TRY(ctx, hipMemcpy(cnt.data(), A + (par ? s->L.cnt1 : s->L.cnt0), sizeof(int32_t) * N, hipMemcpyDeviceToHost));
-        HIP_TRY(ctx, hipMemcpy(ent.data(), A + (par ? s->L.stale1 : s->L.stale0), sizeof(QpsoStale) * ent.size(), hipMemcpyDeviceToHost));
+        HIP_TRY(ctx, copy_on(ctx->stream, cnt.data(), A + (par ? s->L.cnt1 : s->L.cnt0), sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+        HIP_TRY(ctx, copy_on(ctx->stream, ent.data(), A + (par ? s->L.stale1 : s->L.stale0), sizeof(QpsoStale) * ent.size(), hipMemcpyDeviceToHost));
         for (int p = 0; p < N; ++p) {
             int32_t* fr = tr->frames + (size_t)p * kFrame;
             std::fill(fr, fr + kFrame, (int32_t)QPSO_EMPTY);
@@ -906,7 +987,7 @@ int qpso_autotune(qpso_ctx* ctx, int n, qpso_solve* const* solves, int iters, qp
         cleanup();
         return set_err(ctx, QPSO_E_NOMEM, "autotune scratch allocation failed");
     }
-    hipError_t e = hipMemcpy(d_items, items.data(), sizeof(QpsoItem) * items.size(), hipMemcpyHostToDevice);
+    hipError_t e = copy_on(st, d_items, items.data(), sizeof(QpsoItem) * items.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     auto run = [&](int cfg) {
@@ -922,7 +1003,7 @@ int qpso_autotune(qpso_ctx* ctx, int n, qpso_solve* const* solves, int iters, qp
     constexpr int kNBig = (int)(sizeof(kBigFaces) / sizeof(kBigFaces[0]));
     for (int b = 0; b < kNBig && e == hipSuccess; ++b) {
         for (auto& d : descs) d.big_face = kBigFaces[b];
-        e = hipMemcpy(d_descs, descs.data(), sizeof(QpsoSolveDev) * n, hipMemcpyHostToDevice);
+        e = copy_on(st, d_descs, descs.data(), sizeof(QpsoSolveDev) * n, hipMemcpyHostToDevice);
         if (b == 0 && e == hipSuccess) e = run(0);  // warm-up
         for (int c = 0; c < n_cfg && e == hipSuccess; ++c) {
             float ms = 0.0f;
@@ -957,7 +1038,7 @@ int qpso_autotune(qpso_ctx* ctx, int n, qpso_solve* const* solves, int iters, qp
         s->rf_cfg = best_cfg;
         s->h.big_face = best_big;
         drop_solve_graph(s);
-        HIP_TRY(ctx, hipMemcpy(s->d_desc, &s->h, sizeof(QpsoSolveDev), hipMemcpyHostToDevice));
+        HIP_TRY(ctx, copy_on(ctx->stream, s->d_desc, &s->h, sizeof(QpsoSolveDev), hipMemcpyHostToDevice));
     }
     return QPSO_OK;
 }
@@ -999,8 +1080,8 @@ int qpso_solve_run_batch(qpso_ctx* ctx, int n, qpso_solve* const* solves, qpso_r
         P.n_items = (int)items.size();
         HIP_TRY(ctx, hipMalloc(&P.d_descs, sizeof(QpsoSolveDev) * n));
         HIP_TRY(ctx, hipMalloc(&P.d_items, sizeof(QpsoItem) * items.size()));
-        HIP_TRY(ctx, hipMemcpy(P.d_descs, descs.data(), sizeof(QpsoSolveDev) * n, hipMemcpyHostToDevice));
-        HIP_TRY(ctx, hipMemcpy(P.d_items, items.data(), sizeof(QpsoItem) * items.size(), hipMemcpyHostToDevice));
+        HIP_TRY(ctx, copy_on(ctx->stream, P.d_descs, descs.data(), sizeof(QpsoSolveDev) * n, hipMemcpyHostToDevice));
+        HIP_TRY(ctx, copy_on(ctx->stream, P.d_items, items.data(), sizeof(QpsoItem) * items.size(), hipMemcpyHostToDevice));
         hipGraph_t graph = nullptr;
         HIP_TRY(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         hipError_t e = qpso_launch::init(P.d_descs, n, st);
@@ -1052,6 +1133,10 @@ struct HipDev {
         std::vector<uint16_t> depth;  // the caller's frame, copied at submit
     };
     static constexpr int kInFlight = 2;
+    // Two lanes (requests join them by size).  The lanes run side by side only on
+    // hardware queues of their own: the process has four by default, handed
+    // to the least used stream first; the context's stream holds one and,
+    // once a graph has been launched, the runtime another.
     static constexpr int kLanes = 2;
     struct Lane {
         hipStream_t st = nullptr;
@@ -1066,6 +1151,19 @@ struct HipDev {
     qpso_ctx* ctx;
     int rf_cfg, big_face;  // the context's tuning when the server was created
     Lane lanes[kLanes];
+    // A finished solve's last step: its result words, copied into pinned
+    // host memory on its lane right after that step, and the event after
+    // the copies.  Reading a result then needs no call into the runtime.
+    struct End {
+        hipEvent_t ev;
+        ResultWords* words;
+    };
+    std::unordered_map<qpso_solve*, End> ends;
+    std::vector<hipEvent_t> spare;       // events to reuse
+    std::vector<ResultWords*> free_words;  // pinned result slots to reuse
+    std::vector<End> draining;             // a cancelled solve's copies may still be queued
+    std::vector<ResultWords*> blocks;      // their allocations (kBlock slots each)
+    static constexpr size_t kBlock = 256;
 
     explicit HipDev(qpso_ctx* c) : ctx(c), rf_cfg(c->rf_cfg), big_face(c->big_face) {}
 
@@ -1089,6 +1187,15 @@ struct HipDev {
             if (l.st) hipStreamDestroy(l.st);
             l = Lane{};
         }
+        for (auto& kv : ends) spare.push_back(kv.second.ev);
+        ends.clear();
+        for (auto& d : draining) spare.push_back(d.ev);
+        draining.clear();
+        for (hipEvent_t e : spare) hipEventDestroy(e);
+        spare.clear();
+        free_words.clear();
+        for (ResultWords* b : blocks) hipHostFree(b);
+        blocks.clear();
     }
     void on_thread_start() { hipSetDevice(ctx->device); }
     void on_thread_end() {}
@@ -1148,6 +1255,11 @@ struct HipDev {
             return QPSO_E_HIP;
         return QPSO_OK;
     }
+    // the lane's oldest queued step of the last kInFlight has completed
+    bool ready(int k) {
+        const Lane& l = lanes[k];
+        return l.launched < (uint64_t)kInFlight || hipEventQuery(l.ev[l.launched % kInFlight]) != hipErrorNotReady;
+    }
     // A step with no more workgroups than CUs is latency-bound: it takes the
     // widest configuration (one 1024-thread workgroup per CU); fuller steps
     // take the server's tuned one.  Results do not depend on it.
@@ -1167,9 +1279,68 @@ struct HipDev {
         ++l.launched;
         return QPSO_OK;
     }
+    int mark(int k, qpso_solve* s) {
+        // slots whose copies have landed can be used again
+        for (size_t i = 0; i < draining.size();) {
+            if (hipEventQuery(draining[i].ev) == hipErrorNotReady) {
+                ++i;
+                continue;
+            }
+            spare.push_back(draining[i].ev);
+            free_words.push_back(draining[i].words);
+            draining[i] = draining.back();
+            draining.pop_back();
+        }
+        if (free_words.empty()) {
+            ResultWords* b = nullptr;
+            if (hipHostMalloc(reinterpret_cast<void**>(&b), sizeof(ResultWords) * kBlock) != hipSuccess) return QPSO_E_NOMEM;
+            blocks.push_back(b);
+            for (size_t i = 0; i < kBlock; ++i) free_words.push_back(b + i);
+        }
+        End end{nullptr, free_words.back()};
+        free_words.pop_back();
+        if (!spare.empty()) {
+            end.ev = spare.back();
+            spare.pop_back();
+        } else if (hipEventCreateWithFlags(&end.ev, hipEventDisableTiming) != hipSuccess) {
+            free_words.push_back(end.words);
+            return QPSO_E_HIP;
+        }
+        ends[s] = end;
+        if (copy_result(s, end.words, lanes[k].st) != hipSuccess) return QPSO_E_HIP;
+        return hipEventRecord(end.ev, lanes[k].st) == hipSuccess ? QPSO_OK : QPSO_E_HIP;
+    }
+    int done(qpso_solve* s) {
+        auto it = ends.find(s);
+        if (it == ends.end()) return QPSO_E_HIP;
+        const hipError_t e = hipEventQuery(it->second.ev);
+        return e == hipSuccess ? 1 : e == hipErrorNotReady ? 0 : QPSO_E_HIP;
+    }
     int sync(int k) { return hipStreamSynchronize(lanes[k].st) == hipSuccess ? QPSO_OK : QPSO_E_HIP; }
-    int read(int k, qpso_solve* s, qpso_result* out) { return read_result_status(s, out, lanes[k].st); }
-    void destroy(qpso_solve* s) { qpso_solve_destroy(s); }
+    // after done(): the result words are in host memory
+    int read(qpso_solve* s, qpso_result* out) {
+        auto it = ends.find(s);
+        if (it == ends.end()) return QPSO_E_HIP;
+        return result_from(*it->second.words, out);
+    }
+    // the solve's work all lies on lane k: its memory is released in that
+    // lane's order, with no wait
+    void destroy(qpso_solve* s, int k) {
+        auto it = ends.find(s);
+        if (it != ends.end()) {
+            if (hipEventQuery(it->second.ev) == hipErrorNotReady) {
+                draining.push_back(it->second);
+            } else {
+                spare.push_back(it->second.ev);
+                free_words.push_back(it->second.words);
+            }
+            ends.erase(it);
+        }
+        std::lock_guard<std::mutex> g(ctx->mu);
+        hipSetDevice(ctx->device);
+        solve_release(s, lanes[k].st);
+    }
+    void idle_wait() { std::this_thread::sleep_for(std::chrono::microseconds(20)); }
 };
 
 }  // namespace
@@ -1222,6 +1393,17 @@ int qpso_server_submit(qpso_server* sv, uint32_t mesh_id, const qpso_cluster* cl
     r.table_bytes = sizeof(float) * (uint64_t)S * (uint64_t)p->n_particles;
     r.particles = p->n_particles;
     r.iters = p->n_iters;
+    {
+        // a step's work: particles x faces rendered (an unknown mesh fails at
+        // admission)
+        qpso_ctx* ctx = sv->dev.ctx;
+        std::lock_guard<std::mutex> g(ctx->mu);
+        auto mit = ctx->meshes.find(mesh_id);
+        const int64_t F = mit == ctx->meshes.end() ? 1
+                          : p->face_cap > 0 ? std::min<int64_t>(mit->second.F, p->face_cap)
+                                            : (int64_t)mit->second.F;
+        r.cost = (int64_t)p->n_particles * std::max<int64_t>(F, 1);
+    }
     return sv->core.submit(std::move(r), ticket);
 }
 

@@ -3,30 +3,47 @@
 // device.  No counterpart in the reference, which solves one cluster per
 // process (README.md:26-29).
 //
-// Server<Dev> owns the request queue, admission, the two lanes of resident
+// Server<Dev> owns the request queue, admission, the lanes of resident
 // solves, cancellation and time-outs, and the ticket bookkeeping; Dev does
 // the device work.  The library instantiates it with its HIP device
 // (qpso_api.cpp); the CPU tests instantiate it with a mock device (virtual
 // clock, fake allocator, injected failures).  Dev provides:
 //
+//   kLanes                     lanes of resident solves (queues of the device
+//                              that run side by side)
 //   typename Solve, Job        a resident solve; what a request carries
 //   double now()               seconds on the clock latencies are measured with
 //   int  create(Job&, Solve*&) allocate and upload one solve (status code)
 //   int  init(Solve*, lane)    enqueue its swarm initialisation on the lane
+//   bool ready(lane)           a step can be enqueued without waiting
 //   int  step(lane, solves, changed, total_particles)
 //                              enqueue one iteration of every solve of the lane
-//                              (`changed`: the lane's set differs from the last step)
+//                              (`changed`: the lane's set differs from the last
+//                              step; `total_particles`: over every lane)
+//   int  mark(lane, Solve*)    after the solve's last step: note where its work ends
+//   int  done(Solve*)          1 once that work has completed, 0 before, < 0: error
+//   int  read(Solve*, qpso_result*)   the finished solve's result (after done)
 //   int  sync(lane)            wait until the lane's queued work is done
-//   int  read(lane, Solve*, qpso_result*)   the finished solve's result (after sync)
-//   void destroy(Solve*)       free it (after sync)
+//   void destroy(Solve*, lane) free it once the work queued on the lane (all
+//                              the work that uses it) is done, without waiting
+//   void idle_wait()           wait a little for the device (nothing else to do)
 //   void on_thread_start(), on_thread_end()   hooks of the server thread
 //
+// Requests join lanes by size (choose_lane): like sizes share the lanes, far
+// lighter ones keep to a lane of their own.  Each lane is stepped as soon as
+// it can take a step, independently of the others, so light solves advance
+// at their own step time rather than at that of the largest resident ones.
+// A finished solve's result is posted once its own last step has completed;
+// its memory is released at once, in the order of its lane's queue, and its
+// charge with it.
+//
 // Invariants kept (and checked by the mock-device tests): requests are
-// admitted in arrival order; the device bytes charged (solve arenas plus each
-// random table once while any resident solve uses it) never exceed the
-// budget; resident particles never exceed the slots except for one solve
-// larger than the slots, which then runs alone; every ticket ends with
-// exactly one result, read exactly once.
+// admitted in arrival order; the device bytes charged (solve arenas plus
+// each random table once while any resident solve uses it) never exceed the
+// budget; resident particles never exceed the slots except for
+// one solve larger than the slots, which then runs alone; every ticket ends
+// with exactly one result, read exactly once; nothing is read before the
+// work it depends on completed, nor freed except in its lane's order.
 #ifndef QPSO_SCHED_H_
 #define QPSO_SCHED_H_
 
@@ -57,7 +74,10 @@ class Server {
    public:
     using Solve = typename Dev::Solve;
     using Job = typename Dev::Job;
-    static constexpr int kLanes = 2;
+    static constexpr int kLanes = Dev::kLanes;
+    // requests whose step work differs by more than this factor run in
+    // different lanes where they can
+    static constexpr int64_t kRatio = 64;
 
     struct Request {
         Job job;
@@ -65,6 +85,7 @@ class Server {
         TableKey table{};          // its random table
         uint64_t table_bytes = 0;  // charged once while any resident solve uses the table
         int64_t particles = 0;
+        int64_t cost = 0;          // work of one step: particles x faces rendered
         int32_t iters = 0;
         uint64_t ticket = 0;
         double t_submit = 0.0;
@@ -215,12 +236,14 @@ class Server {
     std::vector<uint64_t> admission_order() { std::lock_guard<std::mutex> lk(mu_); return admitted_; }
 
     // One pass of the loop: expire and admit queued requests, end cancelled
-    // and expired resident solves, step every lane once, finish the solves
-    // that ran all their iterations.  Returns false when there is nothing to
-    // do.  Only one thread may call it (the server thread, or a test).
+    // and expired resident solves, step every lane that can take a step,
+    // post the results of solves whose work has completed, free finished
+    // solves when due.  Returns false when there is nothing to do.  Only one
+    // thread may call it (the server thread, or a test).
     bool tick() {
         std::vector<Request> admit;
         std::unordered_set<uint64_t> cancels;
+        bool progress = false;
         const double now = dev_.now();
         {
             std::lock_guard<std::mutex> lk(mu_);
@@ -251,103 +274,115 @@ class Server {
             cancels.swap(cancel_);
         }
         for (auto& r : admit) {
+            progress = true;
             Solve* s = nullptr;
             int rc = dev_.create(r.job, s);
-            const int lane = lanes_[0].particles <= lanes_[1].particles ? 0 : 1;
+            const int lane = choose_lane(r);
             if (rc == QPSO_OK) {
                 rc = dev_.init(s, lane);
-                if (rc != QPSO_OK) dev_.destroy(s);
+                if (rc != QPSO_OK) dev_.destroy(s, lane);
             }
             if (rc != QPSO_OK) {
-                finish(r, result_with(rc), kRejected);
+                post(r, result_with(rc), kRejected);
+                release(r);
                 continue;
             }
             Lane& l = lanes_[lane];
             l.particles += r.particles;
+            l.cost += r.cost;
             l.active.push_back(Active{std::move(r), s, 0});
             l.changed = true;
         }
-        // cancelled and expired solves leave at this step boundary
+        // cancelled and expired solves leave at this step boundary; their
+        // memory is freed with the other finished solves
         for (int k = 0; k < kLanes; ++k) {
             Lane& l = lanes_[k];
-            bool any = false;
-            for (auto& a : l.active)
-                any |= cancels.count(a.req.ticket) || (a.req.deadline > 0.0 && now >= a.req.deadline);
-            if (!any) continue;
-            if (const int rc = dev_.sync(k)) {
-                fail_lane(k, rc);
-                continue;
-            }
-            std::vector<Active> keep;
-            for (auto& a : l.active) {
-                const bool cancelled = cancels.count(a.req.ticket) != 0;
-                if (!cancelled && !(a.req.deadline > 0.0 && now >= a.req.deadline)) {
-                    keep.push_back(std::move(a));
-                    continue;
+            auto leaving = [&](const Active& a) {
+                return cancels.count(a.req.ticket) != 0 || (a.req.deadline > 0.0 && now >= a.req.deadline);
+            };
+            for (auto* v : {&l.active, &l.finishing}) {
+                std::vector<Active> keep;
+                for (auto& a : *v) {
+                    if (!leaving(a)) {
+                        keep.push_back(std::move(a));
+                        continue;
+                    }
+                    const bool cancelled = cancels.count(a.req.ticket) != 0;
+                    post(a.req, result_with(cancelled ? QPSO_E_CANCELLED : QPSO_E_TIMEOUT),
+                         cancelled ? kCancelled : kTimedOut);
+                    leave_lane(k, a, v == &l.active);
+                    progress = true;
                 }
-                dev_.destroy(a.s);
-                l.particles -= a.req.particles;
-                finish(a.req, result_with(cancelled ? QPSO_E_CANCELLED : QPSO_E_TIMEOUT),
-                       cancelled ? kCancelled : kTimedOut);
+                if (keep.size() != v->size()) l.changed = true;
+                v->swap(keep);
             }
-            l.active.swap(keep);
-            l.changed = true;
         }
-        size_t n_active = 0;
-        int64_t total = 0;
-        for (auto& l : lanes_) {
-            n_active += l.active.size();
-            total += l.particles;
-        }
-        publish_active(n_active);
-        if (n_active == 0) return !admit.empty();
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            ++steps_;
-            active_sum_ += (double)n_active;
-        }
+        // step every lane that can take a step now
+        size_t stepped = 0;
+        int64_t all_particles = 0;  // over every lane: what the device has to run
+        for (const auto& l : lanes_) all_particles += l.particles;
         for (int k = 0; k < kLanes; ++k) {
             Lane& l = lanes_[k];
-            if (l.active.empty()) continue;
+            if (l.active.empty() || !dev_.ready(k)) continue;
             std::vector<Solve*> ss;
             ss.reserve(l.active.size());
             for (auto& a : l.active) ss.push_back(a.s);
-            if (const int rc = dev_.step(k, ss, l.changed, total)) {
+            if (const int rc = dev_.step(k, ss, l.changed, all_particles)) {
                 fail_lane(k, rc);
                 continue;
             }
+            progress = true;
+            stepped += l.active.size();
             l.changed = false;
-            for (auto& a : l.active) ++a.iter;
+            // solves that have run all their iterations wait for their last
+            // step to complete
+            std::vector<Active> keep;
+            int rc = QPSO_OK;
+            for (auto& a : l.active) {
+                if (++a.iter < a.req.iters) {
+                    keep.push_back(std::move(a));
+                    continue;
+                }
+                if (rc == QPSO_OK) rc = dev_.mark(k, a.s);
+                l.particles -= a.req.particles;
+                l.cost -= a.req.cost;
+                l.finishing.push_back(std::move(a));
+                l.changed = true;
+            }
+            l.active.swap(keep);
+            if (rc != QPSO_OK) fail_lane(k, rc);
         }
+        if (stepped) {
+            std::lock_guard<std::mutex> lk(mu_);
+            ++steps_;
+            active_sum_ += (double)resident_solves_loop();
+        }
+        // results of the solves whose last step has completed
         for (int k = 0; k < kLanes; ++k) {
             Lane& l = lanes_[k];
-            bool any_done = false;
-            for (auto& a : l.active) any_done |= a.iter >= a.req.iters;
-            if (!any_done) continue;
-            if (const int rc = dev_.sync(k)) {
-                fail_lane(k, rc);
-                continue;
-            }
             std::vector<Active> keep;
-            for (auto& a : l.active) {
-                if (a.iter < a.req.iters) {
+            int err = QPSO_OK;
+            for (auto& a : l.finishing) {
+                const int d = err ? 0 : dev_.done(a.s);
+                if (d < 0) err = d;
+                if (d != 1) {
                     keep.push_back(std::move(a));
                     continue;
                 }
                 qpso_result res{};
-                const int rc = dev_.read(k, a.s, &res);
+                const int rc = dev_.read(a.s, &res);
                 if (rc != QPSO_OK && res.status == QPSO_OK) res.status = rc;
-                dev_.destroy(a.s);
-                l.particles -= a.req.particles;
-                finish(a.req, res, kCompleted);
+                post(a.req, res, kCompleted);
+                leave_lane(k, a, false);
+                progress = true;
             }
-            l.active.swap(keep);
-            l.changed = true;
+            l.finishing.swap(keep);
+            if (err) fail_lane(k, err);
         }
-        size_t left = 0;
-        for (auto& l : lanes_) left += l.active.size();
-        publish_active(left);
-        return true;
+        const size_t resident = resident_solves_loop();
+        publish_active(resident);
+        if (!progress && resident > 0) dev_.idle_wait();
+        return progress || resident > 0;
     }
 
    private:
@@ -358,8 +393,10 @@ class Server {
         int32_t iter;  // iterations enqueued so far
     };
     struct Lane {
-        std::vector<Active> active;
-        int64_t particles = 0;
+        std::vector<Active> active;     // stepped
+        std::vector<Active> finishing;  // all iterations enqueued, result not yet read
+        int64_t particles = 0;          // of the active solves
+        int64_t cost = 0;
         bool changed = true;
     };
 
@@ -367,6 +404,63 @@ class Server {
         qpso_result r{};
         r.status = status;
         return r;
+    }
+
+    size_t resident_solves_loop() const {
+        size_t n = 0;
+        for (const auto& l : lanes_) n += l.active.size() + l.finishing.size();
+        return n;
+    }
+
+    // A request joins a lane whose resident requests' step work is within a
+    // factor kRatio of its own both ways (or an empty one), the least loaded
+    // of those: requests of like size share the lanes, so both keep the
+    // device busy, while a light request never waits on the steps of a far
+    // heavier one.  When no lane qualifies, the lane whose heaviest resident
+    // is closest to it in ratio.
+    int choose_lane(const Request& r) const {
+        const double c = (double)std::max<int64_t>(r.cost, 1);
+        int best = -1, nearest = 0;
+        double best_cost = 0.0, nearest_gap = 0.0;
+        for (int k = 0; k < kLanes; ++k) {
+            const Lane& l = lanes_[k];
+            double lo = 0.0, hi = 0.0, total = 0.0;
+            bool any = false;
+            for (const auto* v : {&l.active, &l.finishing}) {
+                for (const auto& a : *v) {
+                    const double x = (double)std::max<int64_t>(a.req.cost, 1);
+                    lo = any ? std::min(lo, x) : x;
+                    hi = any ? std::max(hi, x) : x;
+                    any = true;
+                }
+            }
+            for (const auto& a : l.active) total += (double)a.req.cost;
+            if (!any || (lo * (double)kRatio >= c && c * (double)kRatio >= hi)) {
+                if (best < 0 || total < best_cost) {
+                    best = k;
+                    best_cost = total;
+                }
+            }
+            const double gap = any ? std::max(hi / c, c / hi) : 0.0;
+            if (k == 0 || gap < nearest_gap) {
+                nearest = k;
+                nearest_gap = gap;
+            }
+        }
+        return best >= 0 ? best : nearest;
+    }
+
+    // A solve leaves its lane (its result is posted): its slots are free and
+    // its memory is released in the lane's order.  `stepped`: it was still
+    // among the lane's active solves.
+    void leave_lane(int k, Active& a, bool stepped) {
+        Lane& l = lanes_[k];
+        if (stepped) {
+            l.particles -= a.req.particles;
+            l.cost -= a.req.cost;
+        }
+        dev_.destroy(a.s, k);
+        release(a.req);
     }
 
     void run() {
@@ -385,7 +479,7 @@ class Server {
         for (;;) {
             {
                 std::lock_guard<std::mutex> lk(mu_);
-                if (n_active_ == 0 && cancel_.empty()) return;
+                if (n_active_ == 0 && cancel_.empty()) break;
             }
             tick();
         }
@@ -396,30 +490,40 @@ class Server {
         n_active_ = n;
     }
 
-    // Every solve of the lane ends with `code` (a failed launch, copy or sync).
+    // Every solve of the lane ends with `code` (a failed launch, copy, sync
+    // or completion query).
     void fail_lane(int k, int code) {
         Lane& l = lanes_[k];
         dev_.sync(k);
-        for (auto& a : l.active) {
-            dev_.destroy(a.s);
-            finish(a.req, result_with(code), kRejected);
+        for (auto* v : {&l.active, &l.finishing}) {
+            for (auto& a : *v) {
+                dev_.destroy(a.s, k);
+                post(a.req, result_with(code), kRejected);
+                release(a.req);
+            }
+            v->clear();
         }
-        l.active.clear();
         l.particles = 0;
+        l.cost = 0;
         l.changed = true;
     }
 
-    // Release an admitted request's charge and post its result.
-    void finish(const Request& r, const qpso_result& res, Kind kind) {
+    // Post an admitted request's result; its particle slots are free.
+    void post(const Request& r, const qpso_result& res, Kind kind) {
+        std::lock_guard<std::mutex> lk(mu_);
+        resident_ -= r.particles;
+        post_locked(r.ticket, res, kind, r.t_submit);
+    }
+
+    // Release an admitted request's device bytes (its memory is released).
+    void release(const Request& r) {
         std::lock_guard<std::mutex> lk(mu_);
         used_ -= r.bytes;
-        resident_ -= r.particles;
         auto tr = table_refs_.find(r.table);
         if (tr != table_refs_.end() && --tr->second == 0) {
             used_ -= r.table_bytes;
             table_refs_.erase(tr);
         }
-        post_locked(r.ticket, res, kind, r.t_submit);
     }
 
     void post_locked(uint64_t ticket, const qpso_result& res, Kind kind, double t_submit) {

@@ -5,6 +5,7 @@
 // tests/test_scheduler.py (with AddressSanitizer/UBSan, and ThreadSanitizer
 // for the threaded scenario).  Prints one line per scenario; exits non-zero
 // on the first violated check.
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -32,8 +33,10 @@ int g_failures = 0;
     } while (0)
 
 struct MockDev {
+    static constexpr int kLanes = 2;  // as the library
     struct Job {
         int id = 0;
+        int64_t particles = 0;
         bool fail_create = false;  // the allocator refuses this request
         bool fail_init = false;    // its initialisation launch fails
     };
@@ -41,16 +44,32 @@ struct MockDev {
         Job job;
         int lane = -1;
         int steps = 0;
+        double last_use = 0.0;  // completion time of the last queued work that reads it
+        double done_at = -1.0;  // completion of its last step, once marked
     };
+    // Each lane is a queue of its own: a step starts when the lane's previous
+    // one has completed (and not before it is enqueued) and costs step_base
+    // plus step_per_particle per particle of the lane.
     std::atomic<double> clock{0.0};  // virtual seconds
     double step_base = 1e-4, step_per_particle = 1e-7;
     int fail_step_at = -1;  // the n-th step call fails (a failed launch)
     int n_steps = 0;
     std::set<Solve*> alive;
-    bool unsynced[2] = {false, false};
+    std::vector<double> queued[kLanes];  // completion times of the lanes' queued work
+    double lane_end[kLanes] = {};
     int violations = 0;  // device-use errors seen by the mock
+    std::vector<int> lane_of_job = std::vector<int>(1024, -1);
 
     double now() const { return clock.load(); }
+    void prune(int lane) {
+        auto& q = queued[lane];
+        q.erase(std::remove_if(q.begin(), q.end(), [&](double t) { return t <= now(); }), q.end());
+    }
+    double enqueue(int lane, double cost) {
+        lane_end[lane] = std::max(now(), lane_end[lane]) + cost;
+        queued[lane].push_back(lane_end[lane]);
+        return lane_end[lane];
+    }
     int create(Job& j, Solve*& s) {
         if (j.fail_create) return QPSO_E_NOMEM;
         s = new Solve{j};
@@ -60,35 +79,62 @@ struct MockDev {
     int init(Solve* s, int lane) {
         if (s->job.fail_init) return QPSO_E_HIP;
         s->lane = lane;
-        unsynced[lane] = true;
+        if (s->job.id >= 0 && s->job.id < (int)lane_of_job.size()) lane_of_job[s->job.id] = lane;
+        s->last_use = enqueue(lane, step_base);
         return QPSO_OK;
     }
-    int step(int lane, const std::vector<Solve*>& ss, bool, int64_t total) {
+    bool ready(int lane) {
+        prune(lane);
+        return queued[lane].size() < 2;
+    }
+    int step(int lane, const std::vector<Solve*>& ss, bool, int64_t) {
         if (n_steps++ == fail_step_at) return QPSO_E_HIP;
+        if (!ready(lane)) ++violations;  // the server steps only lanes that are ready
+        int64_t particles = 0;
+        for (Solve* s : ss) particles += s->job.particles;
+        const double end = enqueue(lane, step_base + step_per_particle * (double)particles);
         for (Solve* s : ss) {
-            if (!alive.count(s) || s->lane != lane) ++violations;
+            if (!alive.count(s) || s->lane != lane || s->done_at >= 0.0) ++violations;
             ++s->steps;
+            s->last_use = end;
         }
-        clock.store(clock.load() + step_base + step_per_particle * (double)total);
-        unsynced[lane] = true;
         return QPSO_OK;
+    }
+    int mark(int lane, Solve* s) {
+        if (s->lane != lane) ++violations;
+        s->done_at = lane_end[lane];
+        return QPSO_OK;
+    }
+    int done(Solve* s) {
+        if (s->done_at < 0.0) ++violations;
+        return now() >= s->done_at ? 1 : 0;
     }
     int sync(int lane) {
-        unsynced[lane] = false;
+        clock.store(std::max(now(), lane_end[lane]));
+        queued[lane].clear();
         return QPSO_OK;
     }
-    int read(int lane, Solve* s, qpso_result* r) {
-        if (unsynced[lane] || s->lane != lane) ++violations;  // results are read only after a sync
+    int read(Solve* s, qpso_result* r) {
+        if (s->done_at < 0.0 || now() < s->done_at) ++violations;  // read only after its work completed
         r->status = QPSO_OK;
         r->fit = (float)s->steps;  // iterations the solve was stepped
         r->raster_px = (uint64_t)s->job.id;
         return QPSO_OK;
     }
-    void destroy(Solve* s) {
+    // released in the lane's order: the lane must be the one that ran it
+    void destroy(Solve* s, int lane) {
         if (!alive.count(s)) ++violations;  // double free
-        if (s->lane >= 0 && unsynced[s->lane]) ++violations;  // freed under queued work
+        if (s->lane >= 0 && s->lane != lane) ++violations;  // not ordered after its work
         alive.erase(s);
         delete s;
+    }
+    // nothing to do until the device moves: advance to its next completion
+    void idle_wait() {
+        double next = -1.0;
+        for (int k = 0; k < kLanes; ++k)
+            for (double t : queued[k])
+                if (t > now() && (next < 0.0 || t < next)) next = t;
+        if (next > 0.0) clock.store(next);
     }
     void on_thread_start() {}
     void on_thread_end() {}
@@ -97,11 +143,13 @@ struct MockDev {
 using Srv = qpso_sched::Server<MockDev>;
 
 Srv::Request req(int id, uint64_t bytes, int64_t particles, int iters, uint32_t table_seed = 1,
-                 uint64_t table_bytes = 0) {
+                 uint64_t table_bytes = 0, int64_t cost = 0) {
     Srv::Request r;
     r.job.id = id;
+    r.job.particles = particles;
     r.bytes = bytes;
     r.particles = particles;
+    r.cost = cost > 0 ? cost : particles;
     r.iters = iters;
     r.table = qpso_sched::TableKey(table_seed, (int32_t)particles, 512);
     r.table_bytes = table_bytes;
@@ -227,8 +275,13 @@ bool cancel_queued_and_resident() {
     CHECK(sv.cancel(b) == QPSO_OK);  // queued: ends at once
     CHECK(sv.poll(b, &r) == QPSO_E_CANCELLED && r.status == QPSO_E_CANCELLED);
     CHECK(sv.cancel(b) == QPSO_E_NOTFOUND);  // already read
-    sv.tick();  // c now fits beside a, runs its one iteration and finishes
-    CHECK(sv.poll(c, &r) == QPSO_OK);
+    // c now fits beside a and runs its one iteration
+    int rc_c = QPSO_E_PENDING;
+    for (int i = 0; i < 100 && rc_c == QPSO_E_PENDING; ++i) {
+        sv.tick();
+        rc_c = sv.poll(c, &r);
+    }
+    CHECK(rc_c == QPSO_OK && r.fit == 1.0f);
     CHECK(sv.cancel(a) == QPSO_OK);  // resident: leaves at the next step boundary
     CHECK(sv.poll(a, &r) == QPSO_E_PENDING);
     sv.tick();
@@ -285,11 +338,13 @@ bool step_failure_ends_one_lane() {
     MockDev dev;
     Srv sv(dev, 1u << 30);
     dev.fail_step_at = 0;  // the first step call: lane 0's first step
+    // like sizes: one in each lane
     const uint64_t a = submit(sv, req(1, 10, 64, 4)), b = submit(sv, req(2, 10, 64, 4));
     if (!drain(sv, dev, 1u << 30, 8192)) return false;
     qpso_result r{};
     CHECK(sv.poll(a, &r) == QPSO_E_HIP);  // lane 0
     CHECK(sv.poll(b, &r) == QPSO_OK && r.fit == 4.0f);  // lane 1 carries on
+    CHECK(dev.lane_of_job[1] == 0 && dev.lane_of_job[2] == 1);
     CHECK(dev.alive.empty());
     return true;
 }
@@ -350,6 +405,104 @@ bool poisson_stream() {
     return true;
 }
 
+// A light request resident beside a heavy one runs in the other lane at
+// its own step time: it finishes long before one iteration of its own at
+// the heavy solve's step time would.  A second heavy request then joins the
+// heavy one's lane, not the light one's.
+bool light_lane_latency() {
+    MockDev dev;
+    Srv sv(dev, 1u << 30);
+    sv.set_slots(1 << 20);
+    // heavy: 100,000 particles, a step of ~10 ms; light: 64 particles, a
+    // step of ~0.1 ms, 20 iterations
+    const uint64_t h1 = submit(sv, req(1, 1000, 100000, 30, 1, 0, 100000LL * 100000));
+    const uint64_t lt = submit(sv, req(3, 100, 64, 20, 3, 0, 64LL * 1000));
+    const uint64_t h2 = submit(sv, req(2, 1000, 100000, 30, 2, 0, 100000LL * 100000));
+    CHECK(h1 && h2 && lt);
+    if (!drain(sv, dev, 1u << 30, 1 << 20)) return false;
+    CHECK(dev.lane_of_job[1] == 0 && dev.lane_of_job[3] == 1 && dev.lane_of_job[2] == 0);
+    qpso_result r{};
+    CHECK(sv.poll(lt, &r) == QPSO_OK && r.fit == 20.0f);
+    const double light_ms = r.latency_ms;
+    CHECK(sv.poll(h1, &r) == QPSO_OK && r.fit == 30.0f);
+    const double heavy_ms = r.latency_ms;
+    CHECK(sv.poll(h2, &r) == QPSO_OK && r.fit == 30.0f);
+    CHECK(heavy_ms > 250.0);  // 30 steps of ~10 ms or more
+    CHECK(light_ms < 10.0);   // under one heavy step
+    std::printf("  light lane: light solve %.2f ms, heavy solve %.1f ms (virtual)\n", light_ms, heavy_ms);
+    CHECK(sv.used_bytes() == 0 && dev.alive.empty() && dev.violations == 0);
+    return true;
+}
+
+// Requests of like size are spread over both lanes by load.
+bool like_sizes_balanced() {
+    MockDev dev;
+    Srv sv(dev, 1u << 30);
+    for (int k = 0; k < 8; ++k) submit(sv, req(10 + k, 10, 100 + 10 * k, 5, 1, 0, 1000 + 100 * k));
+    if (!drain(sv, dev, 1u << 30, 8192)) return false;
+    int in0 = 0;
+    for (int k = 0; k < 8; ++k) in0 += dev.lane_of_job[10 + k] == 0;
+    CHECK(in0 == 4);
+    return true;
+}
+
+// The stress mix in miniature: heavy and light requests interleaved and
+// submitted at once; every light one finishes before the first heavy one.
+bool interleaved_mix() {
+    MockDev dev;
+    Srv sv(dev, 1u << 30);
+    sv.set_slots(1 << 24);
+    std::vector<uint64_t> heavy, light;
+    for (int k = 0; k < 8; ++k) {
+        heavy.push_back(submit(sv, req(100 + k, 1000, 2048, 50, 10 + k, 0, 2048LL * 100000)));
+        light.push_back(submit(sv, req(200 + k, 100, 64, 20 + 20 * k, 30 + k, 0, 64LL * 1016)));
+    }
+    if (!drain(sv, dev, 1u << 30, 1 << 24)) return false;
+    double light_max = 0.0, heavy_min = 1e30;
+    qpso_result r{};
+    for (uint64_t t : light) {
+        CHECK(sv.poll(t, &r) == QPSO_OK);
+        light_max = std::max(light_max, r.latency_ms);
+    }
+    for (uint64_t t : heavy) {
+        CHECK(sv.poll(t, &r) == QPSO_OK && r.fit == 50.0f);
+        heavy_min = std::min(heavy_min, r.latency_ms);
+    }
+    for (int k = 0; k < 8; ++k) CHECK(dev.lane_of_job[200 + k] == 1 && dev.lane_of_job[100 + k] == 0);
+    std::printf("  interleaved: light max %.1f ms, heavy min %.1f ms (virtual)\n", light_max, heavy_min);
+    CHECK(light_max < heavy_min);
+    CHECK(sv.used_bytes() == 0 && dev.alive.empty() && dev.violations == 0);
+    return true;
+}
+
+// A finished solve's bytes are released with it: a queued request that needs
+// them is admitted while other solves still run.
+bool freed_bytes_admit() {
+    MockDev dev;
+    Srv sv(dev, 1000);
+    const uint64_t lng = submit(sv, req(1, 300, 10, 5000)), a = submit(sv, req(2, 600, 10, 2)),
+                   b = submit(sv, req(3, 600, 10, 2));
+    CHECK(lng && a && b);
+    qpso_result r{};
+    int rc_b = QPSO_E_PENDING;
+    bool b_beside_long = false;
+    for (int i = 0; i < 100000 && rc_b == QPSO_E_PENDING; ++i) {
+        sv.tick();
+        CHECK(sv.used_bytes() <= 1000 && dev.violations == 0);
+        const auto order = sv.admission_order();
+        if (order.size() == 3 && sv.poll(lng, &r) == QPSO_E_PENDING) b_beside_long = true;
+        rc_b = sv.poll(b, &r);
+    }
+    CHECK(rc_b == QPSO_OK && r.fit == 2.0f);
+    CHECK(b_beside_long);  // a's bytes went to b while the long solve ran
+    CHECK(sv.poll(a, &r) == QPSO_OK && r.fit == 2.0f);
+    CHECK(sv.cancel(lng) == QPSO_OK);
+    if (!drain(sv, dev, 1000, 8192)) return false;
+    CHECK(sv.poll(lng, &r) == QPSO_E_CANCELLED);
+    CHECK(sv.used_bytes() == 0 && dev.alive.empty());
+    return true;
+}
+
 bool threaded_shutdown_releases_waiters() {
     MockDev dev;
     Srv sv(dev, 1000);
@@ -395,6 +548,10 @@ int main(int argc, char** argv) {
         {"create_and_init_failures", create_and_init_failures},
         {"step_failure_ends_one_lane", step_failure_ends_one_lane},
         {"poisson_stream", poisson_stream},
+        {"light_lane_latency", light_lane_latency},
+        {"like_sizes_balanced", like_sizes_balanced},
+        {"interleaved_mix", interleaved_mix},
+        {"freed_bytes_admit", freed_bytes_admit},
         {"threaded_shutdown_releases_waiters", threaded_shutdown_releases_waiters},
     };
     const std::string only = argc > 1 ? argv[1] : "";

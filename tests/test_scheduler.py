@@ -2,12 +2,15 @@
 
 The scheduler (q-pso_amd/csrc/qpso_sched.h) is the policy part of the
 library's serving loop: admission in arrival order under an HBM budget and a
-particle-slot limit, two lanes of resident solves stepped one iteration at a
-time, cancellation, time-outs and ticket bookkeeping.  The same template runs
-here against a mock device with a virtual clock, a fake allocator, injected
-launch failures and seeded Poisson arrivals, built with AddressSanitizer and
-UndefinedBehaviorSanitizer (every scenario) and ThreadSanitizer (the
-scenarios with a server thread and blocked waiters).
+particle-slot limit; two lanes of resident solves that requests join by size
+(like sizes balanced over both, far lighter ones apart), each stepped one
+iteration at a time as soon as it can take a step; results posted when a
+solve's own last step completes; memory released in the order of the lane
+that ran it; cancellation, time-outs and ticket bookkeeping.  The same
+template runs here against a mock device with a virtual clock, a fake
+allocator, injected launch failures and seeded Poisson arrivals, built with
+AddressSanitizer and UndefinedBehaviorSanitizer (every scenario) and
+ThreadSanitizer (the scenarios with a server thread and blocked waiters).
 """
 import os
 import subprocess
@@ -20,7 +23,8 @@ INC = ["-I", os.path.join(REPO, "include"), "-I", os.path.join(REPO, "q-pso_amd"
 
 SCENARIOS = ["fifo_and_budget", "budget_refusal", "table_charged_once", "slots_and_oversize", "iterations_exact",
              "cancel_queued_and_resident", "timeouts", "create_and_init_failures", "step_failure_ends_one_lane",
-             "poisson_stream", "threaded_shutdown_releases_waiters"]
+             "poisson_stream", "light_lane_latency", "like_sizes_balanced", "interleaved_mix", "freed_bytes_admit",
+             "threaded_shutdown_releases_waiters"]
 
 
 def _build(tmp, name, flags):
