@@ -44,12 +44,29 @@ import numpy as np  # noqa: E402
 import qpso  # noqa: E402
 import workloads as W  # noqa: E402
 
-LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz = 78,643 GB/s
+# 256 CUs x 256 B/clk x 2.4 GHz = 157,286 GB/s (gfx950 LDS; 128-bit reads
+# measure 146 TB/s on the device, qpso_peaks)
+LDS_PEAK_GBS = 256 * 256 * 2.4
 HBM_PEAK_GBS = 8000.0
 # 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 VALU instruction per SIMD every 4
 # clocks (16 lanes wide, packed FP32 off)
 VALU_PEAK_TIPS = 256 * 4 * 2.4e9 / 4 / 1e12
 PROFILE = os.path.join(REPO, "profiles", "k_render_fitness_10k_faces_220px_roi_256_particles_pmc.json")
+PEAKS_TOOL = os.path.join(REPO, "q-pso_amd", "lib", "qpso_peaks")
+
+
+def measured_peaks():
+    """The device's own HBM copy, LDS atomic and LDS read ceilings
+    (q-pso_amd/csrc/qpso_peaks.hip, built by `make -C q-pso_amd cal`), or
+    None without the tool."""
+    if not os.path.exists(PEAKS_TOOL):
+        return None
+    import subprocess
+    try:
+        p = subprocess.run([PEAKS_TOOL], capture_output=True, text=True, timeout=120)
+        return json.loads(p.stdout.strip().splitlines()[-1]) if p.returncode == 0 else None
+    except (subprocess.SubprocessError, ValueError, IndexError):
+        return None
 
 
 def pct(v, q):
@@ -355,6 +372,18 @@ def main():
                          "frac": lds_gbs / LDS_PEAK_GBS, "algorithmic_bytes_per_launch": lds_bytes},
         "autotune": {"chosen": tune[0], "candidates": len(tune), "slowest_ms": tune[-1]["ms"]},
     }
+    peaks = measured_peaks()
+    if peaks:
+        out["measured_peaks"] = peaks
+        out["roofline"]["peak_measured"] = peaks["hbm_copy_gbs"]
+        out["roofline"]["frac_measured"] = hbm_gbs / peaks["hbm_copy_gbs"]
+        out["roofline_lds"]["peak_measured"] = peaks["lds_read_gbs"]
+        out["roofline_lds"]["frac_measured"] = lds_gbs / peaks["lds_read_gbs"]
+        # the raster's pixel writes are LDS atomicMin operations, one per P_r
+        atomics = r0.raster_px / params.n_iters / k1_avg_s / 1e9
+        out["roofline_lds_atomic"] = {"bound": "lds-atomic", "achieved": atomics, "peak": peaks["lds_atomic_gops"],
+                                      "unit": "G atomics/s", "frac": atomics / peaks["lds_atomic_gops"],
+                                      "peak_source": "measured on this device (qpso_peaks)"}
     valu = from_profiles("valu_instr_per_launch")
     if valu is not None:
         tips = valu / k1_avg_s / 1e12
