@@ -1114,16 +1114,18 @@ int qpso_solve_run_batch(qpso_ctx* ctx, int n, qpso_solve* const* solves, qpso_r
 namespace {
 
 // The serving loop's device: the scheduler (qpso_sched.h) decides what is
-// resident; this enqueues the work.  One device step of a lane is one PSO
-// iteration of each of its solves: a fused render+fitness launch over all
-// their particles and one PSO launch.  The two lanes have their own streams
-// and work tables, so the GPU overlaps the tail of one lane's step (its last,
-// shortest workgroups) with the other lane's next step.  Within a lane the
-// largest meshes' workgroups start first.  The host counts iterations itself,
-// so steps are enqueued without a round trip; at most kInFlight steps per
-// lane are queued ahead of the device, so a request that arrives joins within
-// about two steps.  A lane's stream is synchronised only when one of its
-// solves leaves.
+// resident and which lane a request joins; this enqueues the work.  One device
+// step of a lane is one PSO iteration of each of its solves: a fused
+// render+fitness launch over all their particles and one PSO launch.  The two
+// lanes have their own streams and work tables; they run side by side, each
+// stepped as soon as it has fewer than kInFlight steps queued (an event
+// query), so the GPU overlaps one lane's step tail with the other's work and
+// a lane of light solves advances at its own pace.  Within a lane the largest
+// meshes' workgroups start first.  The host counts iterations itself, so
+// steps are enqueued without a round trip.  A solve's last step is followed
+// on its lane by the copies of its result words into pinned host memory and
+// an event; its arena goes back to the context's pool in the lane's order.
+// Nothing in the loop waits for the device except a failure path.
 struct HipDev {
     using Solve = qpso_solve;
     struct Job {
