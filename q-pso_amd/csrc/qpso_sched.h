@@ -237,9 +237,10 @@ class Server {
 
     // One pass of the loop: expire and admit queued requests, end cancelled
     // and expired resident solves, step every lane that can take a step,
-    // post the results of solves whose work has completed, free finished
-    // solves when due.  Returns false when there is nothing to do.  Only one
-    // thread may call it (the server thread, or a test).
+    // post the results of solves whose work has completed (their memory is
+    // released in their lane's order as they leave).  Returns false when
+    // there is nothing to do.  Only one thread may call it (the server
+    // thread, or a test).
     bool tick() {
         std::vector<Request> admit;
         std::unordered_set<uint64_t> cancels;
@@ -293,8 +294,7 @@ class Server {
             l.active.push_back(Active{std::move(r), s, 0});
             l.changed = true;
         }
-        // cancelled and expired solves leave at this step boundary; their
-        // memory is freed with the other finished solves
+        // cancelled and expired solves leave at this step boundary
         for (int k = 0; k < kLanes; ++k) {
             Lane& l = lanes_[k];
             auto leaving = [&](const Active& a) {
