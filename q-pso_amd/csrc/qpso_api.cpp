@@ -772,7 +772,14 @@ int qpso_estimate_pose(qpso_ctx* ctx, uint32_t mesh_id, const qpso_cluster* cl, 
     qpso_solve* s = nullptr;
     int r = qpso_solve_create(ctx, mesh_id, cl, p, 0, &s);
     if (r) return r;
-    r = qpso_solve_run(s, out);
+    {
+        // a solve run once: its launches go straight to the stream (capturing
+        // and instantiating a graph would cost more than replaying it saves)
+        std::lock_guard<std::mutex> g(ctx->mu);
+        r = hipSetDevice(ctx->device) == hipSuccess ? launch_solve(s, ctx->stream)
+                                                   : set_err(ctx, QPSO_E_HIP, "hipSetDevice failed");
+        if (!r) r = read_result(s, out, ctx->stream);
+    }
     qpso_solve_destroy(s);
     return r;
 }
