@@ -108,38 +108,53 @@ def host_cpus():
 
 
 def cpu_solve_rate(O, scene, params, i=0, iters=None, threads=1):
-    """Oracle solves/s on `iters` of the solve's iterations, scaled to whole solves."""
+    """Oracle solves/s on `iters` of the solve's iterations, scaled to whole
+    solves; also the seconds taken and the bbox-fill and fitness pixels of
+    the run."""
     op = W.oracle_params(O, scene, params, i)
     if iters is not None:
         op.n_iters = iters
     rand = O.rand_table(params.seed, params.n_particles, params.n_iters)
     t = time.perf_counter()
-    O.solve(scene.mesh_m(i), scene.clusters[i]["depth"], op, rand, n_threads=threads)
+    r = O.solve(scene.mesh_m(i), scene.clusters[i]["depth"], op, rand, n_threads=threads)
     dt = time.perf_counter() - t
-    return (op.n_iters / params.n_iters) / dt, dt
+    return (op.n_iters / params.n_iters) / dt, dt, r["raster_px"], r["visit_px"]
+
+
+def cpu_median(runs):
+    """Median solves/s of timed runs and the pixels/s of the median run."""
+    runs = sorted(runs, key=lambda x: x[0])
+    rate, dt, pr, pf = runs[len(runs) // 2]
+    return rate, dt, {"raster": pr / dt, "fitness": pf / dt}
 
 
 def cpu_baselines(scene1, params1):
     import oracle as O
     O.lib()
     threads, nproc, allowed, model = host_cpus()
-    rate, dt = cpu_solve_rate(O, scene1, params1, threads=threads)
+    cpu_solve_rate(O, scene1, params1, iters=5, threads=threads)  # warm-up
+    rate, dt, px = cpu_median([cpu_solve_rate(O, scene1, params1, threads=threads) for _ in range(5)])
     head = {"value": rate, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": f"one whole solve of the headline workload ({params1.n_particles} particles, "
-                      f"{params1.n_iters} iterations), {dt:.1f} s",
-            "nproc": nproc, "cpus_allowed": allowed, "cpu_model": model}
+            "sample": f"median of 5 whole solves of the headline workload ({params1.n_particles} particles, "
+                      f"{params1.n_iters} iterations), {dt:.1f} s each",
+            "pixels_per_s": px, "nproc": nproc, "cpus_allowed": allowed, "cpu_model": model}
     s0, p0 = W.config0(0)
-    r0s = [cpu_solve_rate(O, s0, p0, threads=threads) for _ in range(3)]
+    cpu_solve_rate(O, s0, p0, threads=threads)  # warm-up
+    r0, _, px0 = cpu_median([cpu_solve_rate(O, s0, p0, threads=threads) for _ in range(5)])
     s2, p2 = W.config2(0)
     idx = [i for i in range(len(p2)) if s2.clusters[i] is not None]
-    t2 = 0.0
+    t2, pr2, pf2, dt_run = 0.0, 0, 0, 0.0
     for i in idx:
-        _, dt2 = cpu_solve_rate(O, s2, p2[i], i, iters=10, threads=threads)
+        _, dt2, pr, pf = cpu_solve_rate(O, s2, p2[i], i, iters=10, threads=threads)
         t2 += dt2 * p2[i].n_iters / 10
+        dt_run += dt2
+        pr2 += pr
+        pf2 += pf
     extra = {
-        "config0": {"value": float(np.median([r for r, _ in r0s])), "unit": "solves/s", "cores": threads,
-                    "sample": "3 whole solves (1,280 faces, 32 particles, 30 iterations), median"},
+        "config0": {"value": r0, "unit": "solves/s", "cores": threads, "pixels_per_s": px0,
+                    "sample": "median of 5 whole solves (1,280 faces, 32 particles, 30 iterations)"},
         "config2_frame": {"value": len(idx) / t2, "unit": "solves/s", "cores": threads,
+                          "pixels_per_s": {"raster": pr2 / dt_run, "fitness": pf2 / dt_run},
                           "sample": f"first 10 of 60 iterations of each of the frame's {len(idx)} solves, "
                                     "scaled to whole solves"},
     }
