@@ -245,25 +245,28 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     //   A vertex takes three words, X and Y read together, then Z; or, packed,
     // one word, X and Y as 16-bit halves, and a drawn face's corner depths
     // are projected again from the face's corner table.  Packing gives the
-    // tile two words per vertex for that re-projection; it is taken (meshes
-    // without windows) when a depth region covering the whole cluster rect
-    // would not fit beside the three-word cache in one band and every 28.4
-    // coordinate of the particle fits the halves (checked after pass A).
+    // tile two words per vertex for that re-projection.  It is taken when
+    // every 28.4 coordinate of the particle fits the halves (checked after
+    // pass A) and, for a cached mesh, a depth region covering the whole
+    // cluster rect would not fit beside the three-word cache in one band.
+    // Windows always pack: a window then holds 2 kWin vertices and the tile
+    // gains kWin words, and a face's cull reads one word per corner (measured
+    // faster even where the band count stays the same).
     const bool cached = !WIN || Ve <= kWin;  // compile-time true without windows
     const bool runs = WIN && !cached && S.n_runs > 0 && S.n_runs * 256 <= Fe;  // runs of 256 faces on average
-    const int VC = cached ? Ve : (runs ? kWin : 0);  // cache capacity in vertices
+    int VC = cached ? Ve : (runs ? kWin : 0);  // cache capacity in vertices (three-word)
     constexpr int kQCap = CAP / 32;   // small faces needing the per-pixel depth
     constexpr int kBQCap = CAP / 128;
     constexpr int kRing = 128;
     constexpr int kFixed = kQCap + 4 * kBQCap + (BT / 64) * kRing + 4;  // queues, rings, alignment
-    bool pack = false;
-    if (!WIN) {
+    bool pack = runs;
+    if (cached) {
         const int32_t rw = S.rect[2] - S.rect[0], rh = S.rect[3] - S.rect[1];
         pack = rw * rh + rh * (((rw + 31) >> 5) + 1) + rw + 8 > CAP - kFixed - 3 * VC;
     }
     uint32_t* __restrict__ vP = reinterpret_cast<uint32_t*>(lds);  // packed: X | Y << 16
     int2* __restrict__ vXY = reinterpret_cast<int2*>(lds);  // X, Y in one 8-byte read
-    float* __restrict__ vZ = reinterpret_cast<float*>(lds + 2 * VC);
+    float* __restrict__ vZ = reinterpret_cast<float*>(lds + 2 * VC);  // set again below for windows
     int32_t vbase = 0;  // vertex index of cache entry 0
 
     // ---- pass A: per-face bbox and AABB over every face (main.h:1340-1353,
@@ -282,8 +285,10 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
                 int32_t X, Y;
                 float Z;
                 dg |= vertex_fixed(pose, x, y, z, cam, X, Y, Z) ? 1 : 0;
-                if (pack) vP[v] = ((uint32_t)X & 0xffffu) | ((uint32_t)Y << 16);  // used only if all fit
-                else if (cached) { vXY[v] = make_int2(X, Y); vZ[v] = Z; }
+                if (cached) {
+                    if (pack) vP[v] = ((uint32_t)X & 0xffffu) | ((uint32_t)Y << 16);  // used only if all fit
+                    else { vXY[v] = make_int2(X, Y); vZ[v] = Z; }
+                }
                 xl = min(xl, X); xh = max(xh, X);
                 yl = min(yl, Y); yh = max(yh, Y);
             };
@@ -334,13 +339,14 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     // (X + 15) >> 4 >= -2047 gives X >= -32767 and <= 2046 gives X <= 32736:
     // every cached vertex fits the 16-bit halves.  A particle with vertices
     // beyond them writes the three-word cache instead (its vertices are still
-    // in registers; the packed words are never read) and takes more bands.
+    // in registers; the packed words are never read) and takes more bands;
+    // with windows it projects them in three words.
     if (pack && !(ax0 >= -2047 && ax1 <= 2046 && ay0 >= -2047 && ay1 <= 2046)) {
         pack = false;
 #pragma unroll
         for (int j = 0; j < kVPT; ++j) {
             const int v = tid + j * BT;
-            if (v < Ve) {
+            if (cached && v < Ve) {
                 int32_t X, Y;
                 float Z;
                 vertex_fixed(pose, ux[j], uy[j], uz[j], cam, X, Y, Z);
@@ -354,6 +360,10 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     // rasterized by a whole wave so one large face does not hold its wave's
     // other 63 lanes idle: clipped
     // rectangle (x0 | x1 << 16, y0 | y1 << 16), constant depth, face << 1 | flat
+    if (WIN && runs) {  // window capacity in vertices
+        VC = pack ? 2 * kWin : kWin;
+        vZ = reinterpret_cast<float*>(lds + 2 * VC);
+    }
     int32_t* __restrict__ queue = lds + (pack ? VC : 3 * VC);
     int32_t* __restrict__ bq_x = queue + kQCap;
     int32_t* __restrict__ bq_y = bq_x + kBQCap;
@@ -438,8 +448,9 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     // face order, e.face == f; passing f lets the corner reads go out without
     // waiting for the entry)
     auto load_face = [&](int32_t f, const QpsoFaceRef& e, Face& fc) {
-        if (!WIN && pack) {
-            const uint32_t pa = vP[e.v0], pb = vP[e.v1], pc = vP[e.v2];
+        if (pack) {
+            const int32_t vb = WIN ? vbase : 0;
+            const uint32_t pa = vP[e.v0 - vb], pb = vP[e.v1 - vb], pc = vP[e.v2 - vb];
             fc.X[0] = (int32_t)(pa << 16) >> 16; fc.Y[0] = (int32_t)pa >> 16;
             fc.X[1] = (int32_t)(pb << 16) >> 16; fc.Y[1] = (int32_t)pb >> 16;
             fc.X[2] = (int32_t)(pc << 16) >> 16; fc.Y[2] = (int32_t)pc >> 16;
@@ -469,9 +480,10 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     // a face to draw: with the packed cache its corner entries carry the
     // vertex indices (w), so the face table entry is not read
     auto draw_face = [&](int32_t f, Face& fc) {
-        if (!WIN && pack) {
+        if (pack) {
             typedef float f4 __attribute__((ext_vector_type(4)));
             const QG f4* __restrict__ cr = (const QG f4*)S.corners + 3 * (size_t)f;
+            const int32_t vb = WIN ? vbase : 0;
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
                 const f4 c = cr[j];
@@ -479,7 +491,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
                 // lowers __builtin_bit_cast of a vector component to the
                 // bits of component 0
                 const float wf = c.w;
-                const uint32_t w = vP[__float_as_int(wf)];
+                const uint32_t w = vP[__float_as_int(wf) - vb];
                 fc.X[j] = (int32_t)(w << 16) >> 16;
                 fc.Y[j] = (int32_t)w >> 16;
                 float u, v;
@@ -793,7 +805,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             int32_t vlo = rt[3 * run + 1], vhi = rt[3 * run + 2];
             f_lo = rt[3 * run];
             int r2 = run + 1;
-            while (r2 < S.n_runs && rt[3 * r2] < Fe && max(vhi, rt[3 * r2 + 2]) - min(vlo, rt[3 * r2 + 1]) <= kWin) {
+            while (r2 < S.n_runs && rt[3 * r2] < Fe && max(vhi, rt[3 * r2 + 2]) - min(vlo, rt[3 * r2 + 1]) <= VC) {
                 vlo = min(vlo, rt[3 * r2 + 1]);
                 vhi = max(vhi, rt[3 * r2 + 2]);
                 ++r2;
@@ -808,7 +820,8 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
                 int32_t X, Y;
                 float Z;
                 vertex_fixed(pose, wx[v], wy[v], wz[v], cam, X, Y, Z);
-                vXY[v - vlo] = make_int2(X, Y); vZ[v - vlo] = Z;
+                if (pack) vP[v - vlo] = ((uint32_t)X & 0xffffu) | ((uint32_t)Y << 16);
+                else { vXY[v - vlo] = make_int2(X, Y); vZ[v - vlo] = Z; }
             }
             __syncthreads();
         }

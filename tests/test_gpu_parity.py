@@ -231,6 +231,33 @@ def test_big_mesh_windows(ctx, face_cap):
     check_solve(*run_solve_pair(ctx, scene, params))
 
 
+def test_big_mesh_windows_far_vertices(ctx):
+    # windows of the 16k-face mesh use the packed (16-bit X, Y) vertex cache;
+    # poses close to the camera put vertices beyond 2047 pixels while part of
+    # the mesh stays in the frame (finite fitness, rendered pixels): those
+    # particles project their windows in three words instead
+    mesh = qpso.mesh_uv_ellipsoid(103, 81, 70.0, 50.0, 35.0, 4, 8.0, 16.0, 5)
+    rng = np.random.default_rng(77)
+    pose = W.gt_pose(rng, xy=60.0, z=(700.0, 900.0), seed=77)
+    depth, labels = qpso.scene_render([(mesh, pose)], W.CAM, 1100.0, 2.0, 0.05, 78)
+    scene = W.Scene([(mesh, pose)], depth, labels)
+    params = qpso.make_params(16, 2, w=0.7, c1=1.5, c2=1.5, cam=W.CAM, seed=0)
+    poses = sample_poses(scene, params.n_particles, seed=23, wild=0.3)
+    gt = scene.objects[0][1]
+    r = np.random.default_rng(31)
+    for i in range(8):
+        q = gt[3:] + r.normal(0, 0.05, 4)
+        q = q / np.linalg.norm(q)
+        poses[:, 2 * i] = [0.03 + 0.005 * i, r.uniform(-0.02, 0.02), 0.06 + 0.005 * i, *q]
+    check_eval(ctx, scene, params, poses)
+    fit = check_eval(ctx, scene, params, poses, frames=False)
+    near = poses[:, 0::2]
+    fo, ao, fr = oracle_eval(scene, params, near)
+    far = ao[:, 1] > 2046
+    assert (far & np.isfinite(fo) & ((fr < 60000).sum(axis=(1, 2)) > 0)).any()
+    assert np.isfinite(fit).sum() > params.n_particles // 2
+
+
 def test_big_mesh_without_runs(ctx):
     # the same 16k-face mesh with its faces shuffled: corners of neighbouring
     # faces are far apart in vertex order, so there are no vertex windows and
