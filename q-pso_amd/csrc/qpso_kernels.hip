@@ -147,24 +147,51 @@ struct Visits {
     }
 };
 
-// WIN: the instantiation that can draw a mesh window by window (larger than
-// the vertex cache); the other one keeps the cached path free of it.
+// Waves per SIMD when as many workgroups as the LDS allows share a CU (160
+// KiB of LDS, 4 SIMDs): the register budget both bodies of a windowed
+// kernel must fit, so that its extra code never costs a workgroup.  Left
+// free (1) for 256-thread workgroups: the 10,112-word tile would need 4 waves
+// and 128 registers, which its bodies exceed without spilling at 3.
+constexpr int rf_waves_per_simd(int bt, int cap) {
+    return bt >= 512 ? (bt / 256) * ((160 * 1024) / (cap * 4 + 512)) : 1;
+}
+
+// The render kernel's LDS, declared once per kernel and shared by the bodies
+// it may run.
+template <int CAP>
+struct RfLds {
+    __attribute__((aligned(16))) int32_t lds[CAP];  // [vertex cache: (X, Y) pairs, Z] [depth tile]
+    int32_t s_box[4];    // rendered AABB: minX, maxX, minY, maxY
+    int32_t s_stale[4];  // carried pixels inside the union box: x0, x1, y0, y1 (inclusive)
+    int32_t s_degen;
+    int32_t s_nstale;
+    int32_t s_nq;        // small non-flat faces queued
+    int32_t s_nqb;       // big faces queued
+    int32_t s_take;      // next big face to hand to a wave
+    int32_t s_later[2];  // a face of the window reaches later bands (by window parity)
+    uint32_t s_cnt[3];   // seg, tr, n_rendered
+    unsigned long long s_px[2];
+};
+
+// WIN: the body that can draw a mesh window by window (larger than the vertex
+// cache); the other one keeps the cached path free of it.
 template <int BT, int CAP, int MODE>
-__global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __restrict__ solves,
-                                                      const QpsoItem* __restrict__ items) {
+__device__ __forceinline__ void rf_body(const QpsoSolveDev* __restrict__ solves, const QpsoItem* __restrict__ items,
+                                        RfLds<CAP>& sh) {
     constexpr bool WIN = MODE >= 1;     // meshes beyond the vertex cache, window by window
     constexpr bool FRAMES = MODE == 2;  // and the frame dump (stage calls)
     constexpr int SPT = 2048 / BT;  // fitness slots per thread
-    __shared__ __attribute__((aligned(16))) int32_t lds[CAP];    // [vertex cache: (X, Y) pairs, Z] [depth tile]
-    __shared__ int32_t s_box[4];    // rendered AABB: minX, maxX, minY, maxY
-    __shared__ int32_t s_stale[4];  // carried pixels inside the union box: x0, x1, y0, y1 (inclusive)
-    __shared__ int32_t s_degen;
-    __shared__ int32_t s_nstale;
-    __shared__ int32_t s_nq;        // small non-flat faces queued
-    __shared__ int32_t s_nqb;       // big faces queued
-    __shared__ int32_t s_take;      // next big face to hand to a wave
-    __shared__ uint32_t s_cnt[3];   // seg, tr, n_rendered
-    __shared__ unsigned long long s_px[2];
+    int32_t* const lds = sh.lds;
+    int32_t* const s_box = sh.s_box;
+    int32_t* const s_stale = sh.s_stale;
+    int32_t& s_degen = sh.s_degen;
+    int32_t& s_nstale = sh.s_nstale;
+    int32_t& s_nq = sh.s_nq;
+    int32_t& s_nqb = sh.s_nqb;
+    int32_t& s_take = sh.s_take;
+    int32_t* const s_later = sh.s_later;
+    uint32_t* const s_cnt = sh.s_cnt;
+    unsigned long long* const s_px = sh.s_px;
 
     const int tid = threadIdx.x;
     // no item table: one solve, workgroup b renders particle b
@@ -232,6 +259,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         s_stale[0] = INT32_MAX; s_stale[1] = INT32_MIN;
         s_stale[2] = INT32_MAX; s_stale[3] = INT32_MIN;
         s_degen = 0; s_nstale = 0; s_nq = 0; s_nqb = 0; s_take = 0;
+        s_later[0] = s_later[1] = 0;
         s_cnt[0] = s_cnt[1] = s_cnt[2] = 0;
         s_px[0] = s_px[1] = 0;
     }
@@ -661,8 +689,14 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
     uint32_t* mask = nullptr;    // [band rows][MW]
     int32_t* col_d = nullptr;    // [band rows]: depth of (minX, y)
     int32_t* row_d = nullptr;    // [TW]: depth of (x, maxY)
-    uint64_t fmask = ~0ull;  // bit k: face entry tid + k*BT still reaches later bands
+    uint64_t fmask = ~0ull;  // bit k: face entry tid + k*BT still reaches later bands (meshes without windows)
     const bool use_mask = !runs && n_faces <= 64 * BT;
+    // windows: bit w, window w of a band (the same windows every band) still
+    // has faces reaching later bands; a window without is neither projected
+    // nor culled again.  wpar: parity of the windows drawn so far, the slot
+    // of s_later the current window uses.
+    uint64_t wmask = ~0ull;
+    int wpar = 0;
     for (int32_t b0 = ty0, b1; has_tile && b0 < ty1; b0 = b1) {
         b1 = band_end(b0);  // band rows [b0, b1)
         const int32_t nb = b1 - b0;
@@ -797,6 +831,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         const int lane = tid & 63;
         const uint64_t below = (1ull << lane) - 1ull;
         int run = 0;  // next run of faces (windowed meshes)
+        int wi = -1;  // window index in this band
         for (;;) {
         int f_lo = 0, f_hi = n_faces;
         if (runs) {
@@ -813,6 +848,11 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             f_hi = r2 < S.n_runs ? rt[3 * r2] : n_faces;
             run = r2;
             vbase = vlo;
+            ++wi;
+            if (wi < 64 && !((wmask >> wi) & 1ull)) {  // uniform: no face of it is left
+                if (run >= S.n_runs || rt[3 * run] >= Fe) break;
+                continue;
+            }
             const QG float* __restrict__ wx = (const QG float*)S.verts;
             const QG float* __restrict__ wy = wx + S.V;
             const QG float* __restrict__ wz = wy + S.V;
@@ -829,6 +869,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         int k = 0;
         QpsoFaceRef nxt{0, 0, 0, INT32_MAX};
         if (f_lo + tid < f_hi) nxt = load_ref(faces, f_lo + tid);
+        if (runs) fmask = 0ull;  // windows: non-zero once a face of the window reaches later bands
         for (int base = f_lo + tid - lane; base < f_hi; base += BT, ++k) {
             const int i = base + lane;  // = f_lo + tid + k * BT
             const QpsoFaceRef e = nxt;  // prefetched one iteration ahead
@@ -845,6 +886,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
                     }
                 }
                 if (use_mask && !later) fmask &= ~(1ull << k);
+                if (runs && later) fmask = 1ull;
             }
             const uint64_t m = __ballot(keep);
             if (keep) ring[(head + cnt + __popcll(m & below)) & (kRing - 1)] = i;
@@ -858,6 +900,7 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             }
         }
         if (lane < cnt) draw(ring[(head + lane) & (kRing - 1)]);
+        if (runs && __ballot(fmask != 0ull) != 0ull && lane == 0) s_later[wpar] = 1;
         __syncthreads();
         // queued small faces: per-pixel depth over their D part, K lanes per
         // face (a power of two, as many as the threads allow, at most 64),
@@ -886,6 +929,13 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
         }
         __syncthreads();
         if (tid == 0) { s_nq = 0; s_nqb = 0; s_take = 0; }
+        if (runs) {
+            // s_later[wpar] was set before the barrier above; the other slot
+            // was last read one window earlier, before two barriers
+            if (wi < 64 && s_later[wpar] == 0) wmask &= ~(1ull << wi);
+            if (tid == 0) s_later[wpar ^ 1] = 0;
+            wpar ^= 1;
+        }
         if (!runs || run >= S.n_runs || S.runs[3 * run] >= Fe) break;
         __syncthreads();  // the counters are reset before the next window's faces
         }
@@ -1138,6 +1188,23 @@ __global__ __launch_bounds__(BT) void k_render_fitness(const QpsoSolveDev* __res
             atomicAdd(&S.counters[1], s_px[1]);
         }
     }
+}
+
+// One workgroup per (solve, particle).  A launch whose solves include a mesh
+// beyond the vertex cache runs the windowed body (MODE 1); its particles of
+// cached meshes take the windowless body, which is faster for them.
+template <int BT, int CAP, int MODE>
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(rf_waves_per_simd(BT, CAP))))
+void k_render_fitness(const QpsoSolveDev* __restrict__ solves, const QpsoItem* __restrict__ items) {
+    __shared__ RfLds<CAP> sh;
+    if (MODE == 1) {
+        const QpsoItem item = items ? items[blockIdx.x] : QpsoItem{0, (int32_t)blockIdx.x};
+        if (solves[item.solve].V_eff <= CAP / 6) {
+            rf_body<BT, CAP, 0>(solves, items, sh);
+            return;
+        }
+    }
+    rf_body<BT, CAP, MODE>(solves, items, sh);
 }
 
 // ------------------------------------------------------------------------
