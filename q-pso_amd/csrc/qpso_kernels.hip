@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 
 #include "qpso_internal.h"
 #include "qpso_kernels.h"
@@ -1023,44 +1024,53 @@ __device__ __forceinline__ void rf_body(const QpsoSolveDev* __restrict__ solves,
             // than x / t for 0 < x < W.  There it is y - 1 when row y is
             // "short", and t visits (0, y - 1); column 0 of the union box lies
             // in Q only when qxa = 0.  A t inside Q in (x, y) is at least tlo
-            // and below k1 W + 1, so of [tlo, thi) only t < T is left to test.
+            // and below k1 W + 1, so of [tlo, thi) only t < T is left to test,
+            // and only in the last batch: every t of an earlier one is below
+            // its end, at most thi.  The four forms (column 0 in Q or not,
+            // last batch or not) are compiled apart.
             const uint32_t xn = (uint32_t)(qxb - qxa), yn = (uint32_t)(qyb - qya);
             const int32_t sq = BT / W, sr = BT - sq * W;
             const bool col0 = qxa == 0;
             // the crop addressed by a 32-bit byte offset from its base
             const QG char* __restrict__ ob = (const QG char*)obs;
-            int32_t t = (tlo / 2048) * 2048 + tid;
-            int32_t y = row_of(t, vis.rW), x = t - y * W;
-            if (x >= W) { x -= W; ++y; }
+            int32_t y, x;
+            {
+                const int32_t t = (tlo / 2048) * 2048 + tid;
+                y = row_of(t, vis.rW);
+                x = t - y * W;
+                if (x >= W) { x -= W; ++y; }
+            }
             // the three counters per wave, from ballots (scalar adds)
             uint32_t w_rend = 0, w_tr = 0, w_seg = 0;
-            for (int32_t base = (tlo / 2048) * 2048; base < thi; base += 2048 * U) {
+            auto batch = [&](int32_t base, auto c0, auto last) {
+                constexpr bool kCol0 = decltype(c0)::value, kLast = decltype(last)::value;
                 int32_t zr[U][SPT], zk[U][SPT];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
 #pragma unroll
                     for (int j = 0; j < SPT; ++j) {
-                        // t = base + u 2048 + j BT + tid, slot j BT + tid; for
-                        // t < thi <= T <= 2^23, y < 2^23 fits the 24-bit products
+                        // t = base + u 2048 + j BT + tid = y W + x, slot j BT +
+                        // tid; for t < thi <= T <= 2^23, y < 2^23 fits the
+                        // 24-bit products
                         int32_t yv = y;
-                        if (col0) {
+                        if (kCol0) {
                             // lanes hold consecutive t: a wave meets x = 0 once
                             // per W values of t, so the fp32 row is formed only then
                             const bool z = x == 0;
                             if (__ballot(z) != 0ull) {
                                 // kept as a branch (not evaluated for every t)
                                 __asm__ volatile("" ::: "memory");
-                                if (z && row_of(t, vis.rW) != y) yv = y - 1;
+                                if (z && row_of(y * W, vis.rW) != y) yv = y - 1;
                             }
                         }
                         zr[u][j] = QPSO_EMPTY;
                         zk[u][j] = 0;
-                        if ((uint32_t)t < (uint32_t)thi && (uint32_t)(x - qxa) < xn && (uint32_t)(yv - qya) < yn) {
+                        if ((!kLast || (uint32_t)(base + u * 2048 + j * BT + tid) < (uint32_t)thi) &&
+                            (uint32_t)(x - qxa) < xn && (uint32_t)(yv - qya) < yn) {
                             zr[u][j] = tile[__mul24(yv, DW) + x + tc];
                             const uint32_t oi = (uint32_t)(__mul24(yv, rw) + x + oc);  // >= 0: a rect pixel
                             zk[u][j] = (int32_t)*(const QG uint16_t*)(ob + 2u * oi);
                         }
-                        t += BT;
                         x += sr;
                         y += sq;
                         if (x >= W) { x -= W; ++y; }
@@ -1080,6 +1090,18 @@ __device__ __forceinline__ void rf_body(const QpsoSolveDev* __restrict__ solves,
                         const float e = (float)((r && o) ? wsub(zr[u][j], zk[u][j]) : 0) * 0.001f;
                         acc_e[j] = __builtin_fmaf(e, e, acc_e[j]);
                     }
+                }
+            };
+            using yes = std::true_type;
+            using no = std::false_type;
+            for (int32_t base = (tlo / 2048) * 2048; base < thi; base += 2048 * U) {
+                const bool last = base + 2048 * U > thi;
+                if (col0) {
+                    if (last) batch(base, yes{}, yes{});
+                    else batch(base, yes{}, no{});
+                } else {
+                    if (last) batch(base, no{}, yes{});
+                    else batch(base, no{}, no{});
                 }
             }
             if ((tid & 63) == 0) {
