@@ -279,12 +279,13 @@ def served_section(ctx, lams=(200.0, 1000.0, 4000.0), seconds=2.5, seed=0, slots
             "served_vs_plain": capacity / plain_rate, "runs": runs, "autotune": tune[0]}
 
 
-def stress_section(ctx, n_stress=64, n_small=64):
-    """The stress load (64 solves of 2,048 particles x 200 iterations on the
-    100k-face mesh, mixed with 64 small 1k-face / 64-particle solves of 20..200
-    iterations) submitted at once under a 32 GiB budget; and the plain rate
-    of the same stress solve (one qpso_estimate_pose call)."""
-    big, small, jobs = W.config4_mix(0, n_stress, n_small)
+def stress_section(ctx, n_stress=64, n_small=64, stress_iters=200):
+    """The stress load (by default 64 solves of 2,048 particles x 200
+    iterations on the 100k-face mesh, mixed with 64 small 1k-face /
+    64-particle solves of 20..200 iterations) submitted at once under a 32 GiB
+    budget; and the plain rate of the same stress solve (one
+    qpso_estimate_pose call)."""
+    big, small, jobs = W.config4_mix(0, n_stress, n_small, stress_iters=stress_iters)
     scenes = [big, small]
     mids = [ctx.upload_mesh(sc.mesh_m()) for sc in scenes]
     js = [(mids[k], scenes[k].cluster(), p) for k, p in jobs]
@@ -301,7 +302,7 @@ def stress_section(ctx, n_stress=64, n_small=64):
         ctx.release_mesh(m)
     rect = big.clusters[0]["rect"]
     return {"workload": f"{n_stress} solves of a {big.objects[0][0].size // 9}-face mesh (ROI "
-                        f"{rect[2] - rect[0]}x{rect[3] - rect[1]}, 2048 particles, 200 iterations, sigma 4 mm, "
+                        f"{rect[2] - rect[0]}x{rect[3] - rect[1]}, 2048 particles, {stress_iters} iterations, sigma 4 mm, "
                         f"15% holes) + {n_small} solves of a {small.objects[0][0].size // 9}-face mesh "
                         f"(64 particles, 20..200 iterations), submitted at once, 32 GiB budget",
             "solves": len(jobs), "all_ok": ok, "solves_per_s": st.solves_per_s,
@@ -325,6 +326,8 @@ def main():
     ap.add_argument("--slots", type=int, default=8192, help="server particle slots (particles rendered per step)")
     ap.add_argument("--served-seconds", type=float, default=2.5, help="Poisson arrivals per rate (s)")
     ap.add_argument("--sat-requests", type=int, default=1500, help="requests of the saturation run")
+    ap.add_argument("--stress-solves", type=int, default=64, help="solves of each kind in the stress mix")
+    ap.add_argument("--stress-iters", type=int, default=200, help="iterations of the stress mix's large solves")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -415,7 +418,7 @@ def main():
         out["served"] = served_section(ctx, lams=[float(x) for x in args.lam.split(",")], seconds=args.served_seconds,
                                        slots=args.slots, n_sat=args.sat_requests)
     if not args.no_stress:
-        out["stress"] = stress_section(ctx)
+        out["stress"] = stress_section(ctx, args.stress_solves, args.stress_solves, args.stress_iters)
     ctx.close()
     print(json.dumps(out))
 
