@@ -297,6 +297,9 @@ __device__ __forceinline__ void rf_body(const QpsoSolveDev* __restrict__ solves,
     int2* __restrict__ vXY = reinterpret_cast<int2*>(lds);  // X, Y in one 8-byte read
     float* __restrict__ vZ = reinterpret_cast<float*>(lds + 2 * VC);  // set again below for windows
     int32_t vbase = 0;  // vertex index of cache entry 0
+    // windows: the vertices [cbase, chi) the cache holds projected (entry
+    // v - cbase), so that a window inside them is not projected again
+    int32_t cbase = 0, chi = 0;
 
     // ---- pass A: per-face bbox and AABB over every face (main.h:1340-1353,
     //      AABBreduction main.h:1190-1304, final reduction main.cu:423-546).
@@ -317,6 +320,9 @@ __device__ __forceinline__ void rf_body(const QpsoSolveDev* __restrict__ solves,
                 if (cached) {
                     if (pack) vP[v] = ((uint32_t)X & 0xffffu) | ((uint32_t)Y << 16);  // used only if all fit
                     else { vXY[v] = make_int2(X, Y); vZ[v] = Z; }
+                } else if (runs && v < 2 * kWin) {
+                    // the packed window from vertex 0, kept if every coordinate fits
+                    vP[v] = ((uint32_t)X & 0xffffu) | ((uint32_t)Y << 16);
                 }
                 xl = min(xl, X); xh = max(xh, X);
                 yl = min(yl, Y); yh = max(yh, Y);
@@ -392,6 +398,7 @@ __device__ __forceinline__ void rf_body(const QpsoSolveDev* __restrict__ solves,
     if (WIN && runs) {  // window capacity in vertices
         VC = pack ? 2 * kWin : kWin;
         vZ = reinterpret_cast<float*>(lds + 2 * VC);
+        if (pack) chi = min(Ve, 2 * kWin);  // pass A's window [0, chi)
     }
     int32_t* __restrict__ queue = lds + (pack ? VC : 3 * VC);
     int32_t* __restrict__ bq_x = queue + kQCap;
@@ -854,17 +861,23 @@ __device__ __forceinline__ void rf_body(const QpsoSolveDev* __restrict__ solves,
                 if (run >= S.n_runs || rt[3 * run] >= Fe) break;
                 continue;
             }
-            const QG float* __restrict__ wx = (const QG float*)S.verts;
-            const QG float* __restrict__ wy = wx + S.V;
-            const QG float* __restrict__ wz = wy + S.V;
-            for (int v = vlo + tid; v < vhi; v += BT) {
-                int32_t X, Y;
-                float Z;
-                vertex_fixed(pose, wx[v], wy[v], wz[v], cam, X, Y, Z);
-                if (pack) vP[v - vlo] = ((uint32_t)X & 0xffffu) | ((uint32_t)Y << 16);
-                else { vXY[v - vlo] = make_int2(X, Y); vZ[v - vlo] = Z; }
+            if (vlo >= cbase && vhi <= chi) {
+                vbase = cbase;  // projected already (pass A, or the last window drawn)
+            } else {
+                const QG float* __restrict__ wx = (const QG float*)S.verts;
+                const QG float* __restrict__ wy = wx + S.V;
+                const QG float* __restrict__ wz = wy + S.V;
+                for (int v = vlo + tid; v < vhi; v += BT) {
+                    int32_t X, Y;
+                    float Z;
+                    vertex_fixed(pose, wx[v], wy[v], wz[v], cam, X, Y, Z);
+                    if (pack) vP[v - vlo] = ((uint32_t)X & 0xffffu) | ((uint32_t)Y << 16);
+                    else { vXY[v - vlo] = make_int2(X, Y); vZ[v - vlo] = Z; }
+                }
+                cbase = vlo;
+                chi = vhi;
+                __syncthreads();
             }
-            __syncthreads();
         }
         int head = 0, cnt = 0;  // wave-uniform ring state
         int k = 0;
