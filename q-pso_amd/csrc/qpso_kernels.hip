@@ -268,9 +268,10 @@ __device__ __forceinline__ void rf_body(const QpsoSolveDev* __restrict__ solves,
 
     // The projected unique vertices of a mesh of at most kWin vertices stay in
     // LDS.  A larger mesh is cut on the host into runs of faces whose corners
-    // lie in a window of at most kWin vertex indices; each band then projects
-    // the windows in turn.  Without usable runs every face's corners are
-    // projected where needed.
+    // lie in a short range of vertex indices; each band then takes the runs
+    // in windows (consecutive runs whose joint range fits the cache) and
+    // projects the windows in turn.  Without usable runs every face's corners
+    // are projected where needed.
     //   A vertex takes three words, X and Y read together, then Z; or, packed,
     // one word, X and Y as 16-bit halves, and a drawn face's corner depths
     // are projected again from the face's corner table.  Packing gives the
