@@ -258,6 +258,31 @@ def test_big_mesh_windows_far_vertices(ctx):
     assert np.isfinite(fit).sum() > params.n_particles // 2
 
 
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4])
+def test_big_mesh_windows_configs(cfg):
+    # the 16k-face mesh (8.2k vertices) in every (threads, tile) variant: one
+    # packed window on the 40,448-word tiles (projected once, in pass A), two
+    # to five windows on the smaller ones, and one to several bands
+    mesh = qpso.mesh_uv_ellipsoid(103, 81, 70.0, 50.0, 35.0, 4, 8.0, 16.0, 5)
+    rng = np.random.default_rng(77)
+    pose = W.gt_pose(rng, xy=60.0, z=(700.0, 900.0), seed=77)
+    depth, labels = qpso.scene_render([(mesh, pose)], W.CAM, 1100.0, 2.0, 0.05, 78)
+    scene = W.Scene([(mesh, pose)], depth, labels)
+    params = qpso.make_params(16, 2, w=0.7, c1=1.5, c2=1.5, cam=W.CAM, seed=0)
+    old = os.environ.get("QPSO_RF_CFG")
+    os.environ["QPSO_RF_CFG"] = str(cfg)
+    try:
+        with qpso.Context(0) as c:
+            poses = sample_poses(scene, params.n_particles, seed=40 + cfg, wild=0.4)
+            check_eval(c, scene, params, poses, frames=False)
+            check_eval(c, scene, params, poses)
+    finally:
+        if old is None:
+            del os.environ["QPSO_RF_CFG"]
+        else:
+            os.environ["QPSO_RF_CFG"] = old
+
+
 def test_big_mesh_without_runs(ctx):
     # the same 16k-face mesh with its faces shuffled: corners of neighbouring
     # faces are far apart in vertex order, so there are no vertex windows and
