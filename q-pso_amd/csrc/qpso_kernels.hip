@@ -169,7 +169,7 @@ struct RfLds {
     int32_t s_nq;        // small non-flat faces queued
     int32_t s_nqb;       // big faces queued
     int32_t s_take;      // next big face to hand to a wave
-    int32_t s_later[2];  // a face of the window reaches later bands (by window parity)
+    int32_t s_later[3];  // a face of the window reaches later bands (slot: windows drawn mod 3)
     uint32_t s_cnt[3];   // seg, tr, n_rendered
     unsigned long long s_px[2];
 };
@@ -260,7 +260,7 @@ __device__ __forceinline__ void rf_body(const QpsoSolveDev* __restrict__ solves,
         s_stale[0] = INT32_MAX; s_stale[1] = INT32_MIN;
         s_stale[2] = INT32_MAX; s_stale[3] = INT32_MIN;
         s_degen = 0; s_nstale = 0; s_nq = 0; s_nqb = 0; s_take = 0;
-        s_later[0] = s_later[1] = 0;
+        s_later[0] = s_later[1] = s_later[2] = 0;
         s_cnt[0] = s_cnt[1] = s_cnt[2] = 0;
         s_px[0] = s_px[1] = 0;
     }
@@ -539,6 +539,12 @@ __device__ __forceinline__ void rf_body(const QpsoSolveDev* __restrict__ solves,
             load_face(f, load_ref(faces, f), fc);
         }
     };
+    // a queued face, drawn after its window (windowed meshes: projected from
+    // the corner table, the same arithmetic as the cache's, so the same bits)
+    auto queued_face = [&](int32_t f, Face& fc) {
+        if (WIN && runs) face_project(S.corners, f, pose, cam, fc);
+        else draw_face(f, fc);
+    };
     // per-pixel depth over [x0, x1) x [y0, y1): edge values at (x0, y0), per
     // pixel CX -= FDY, from a row's last pixel to the next row's first
     // CX += FDX + (w - 1) FDY
@@ -623,7 +629,7 @@ __device__ __forceinline__ void rf_body(const QpsoSolveDev* __restrict__ solves,
         }
         Face fc;
         Setup su;
-        draw_face(fi >> 1, fc);
+        queued_face(fi >> 1, fc);
         face_setup(fc, su);
         int32_t CX2 = wsub(wadd(su.CY2, wmul(y0 + dy - fc.miny, su.FDX23)), wmul(x0 + dx - fc.minx, su.FDY23));
         int32_t CX3 = wsub(wadd(su.CY3, wmul(y0 + dy - fc.miny, su.FDX31)), wmul(x0 + dx - fc.minx, su.FDY31));
@@ -702,8 +708,8 @@ __device__ __forceinline__ void rf_body(const QpsoSolveDev* __restrict__ solves,
     const bool use_mask = !runs && n_faces <= 64 * BT;
     // windows: bit w, window w of a band (the same windows every band) still
     // has faces reaching later bands; a window without is neither projected
-    // nor culled again.  wpar: parity of the windows drawn so far, the slot
-    // of s_later the current window uses.
+    // nor culled again.  wpar: the windows drawn so far mod 3, the slot of
+    // s_later the current window uses.
     uint64_t wmask = ~0ull;
     int wpar = 0;
     for (int32_t b0 = ty0, b1; has_tile && b0 < ty1; b0 = b1) {
@@ -915,8 +921,22 @@ __device__ __forceinline__ void rf_body(const QpsoSolveDev* __restrict__ solves,
             }
         }
         if (lane < cnt) draw(ring[(head + lane) & (kRing - 1)]);
-        if (runs && __ballot(fmask != 0ull) != 0ull && lane == 0) s_later[wpar] = 1;
+        if (!runs) break;  // one pass over the faces; the queues follow
+        // windows: the queued faces wait for the band's end (they are
+        // projected again from the corner table there); every wave is done
+        // with this window's cache before the next one is projected.  The
+        // window's flag goes to slot wpar, read after the barrier; the next
+        // slot (last read two windows ago, before the previous barrier) is
+        // cleared before it.
+        if (__ballot(fmask != 0ull) != 0ull && lane == 0) s_later[wpar] = 1;
+        if (tid == 0) s_later[wpar == 2 ? 0 : wpar + 1] = 0;
         __syncthreads();
+        if (wi < 64 && s_later[wpar] == 0) wmask &= ~(1ull << wi);
+        wpar = wpar == 2 ? 0 : wpar + 1;
+        if (run >= S.n_runs || S.runs[3 * run] >= Fe) break;
+        }
+        __syncthreads();
+        {
         // queued small faces: per-pixel depth over their D part, K lanes per
         // face (a power of two, as many as the threads allow, at most 64),
         // lane j of a face taking its pixels j, j + K, j + 2K, ...
@@ -928,7 +948,7 @@ __device__ __forceinline__ void rf_body(const QpsoSolveDev* __restrict__ solves,
             for (int q = tid / K; q < nq; q += BT / K) {
                 Face fc;
                 Setup su;
-                draw_face(queue[q], fc);
+                queued_face(queue[q], fc);
                 face_setup(fc, su);
                 raster_lanes(fc, su, max(fc.minx, dx0), min(fc.maxx, dx1), max(fc.miny, db0), min(fc.maxy, db1), j, K);
             }
@@ -944,15 +964,6 @@ __device__ __forceinline__ void rf_body(const QpsoSolveDev* __restrict__ solves,
         }
         __syncthreads();
         if (tid == 0) { s_nq = 0; s_nqb = 0; s_take = 0; }
-        if (runs) {
-            // s_later[wpar] was set before the barrier above; the other slot
-            // was last read one window earlier, before two barriers
-            if (wi < 64 && s_later[wpar] == 0) wmask &= ~(1ull << wi);
-            if (tid == 0) s_later[wpar ^ 1] = 0;
-            wpar ^= 1;
-        }
-        if (!runs || run >= S.n_runs || S.runs[3 * run] >= Fe) break;
-        __syncthreads();  // the counters are reset before the next window's faces
         }
 
         if (full) {  // D is the whole tile here
