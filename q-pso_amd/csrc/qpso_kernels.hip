@@ -813,13 +813,16 @@ __device__ __forceinline__ void rf_body(const QpsoSolveDev* __restrict__ solves,
             const int32_t w = px1 - px0, n = __mul24(w, py1 - py0);
             int32_t qb;
             // a non-flat face costs a per-pixel loop in its lane, so it goes to
-            // the wave queue from half the threshold of a flat one
-            if ((flat ? n : 2 * n) > big_face && (qb = atomicAdd(&s_nqb, 1)) < kBQCap) {
+            // the wave queue from half the threshold of a flat one; with that
+            // queue full, to the small-face queue (drawn by several lanes,
+            // per pixel: a flat face's pixels all get kd there too)
+            const bool big = (flat ? n : 2 * n) > big_face;
+            if (big && (qb = atomicAdd(&s_nqb, 1)) < kBQCap) {
                 bq_x[qb] = px0 | (px1 << 16);
                 bq_y[qb] = py0 | (py1 << 16);
                 bq_k[qb] = kd;
                 bq_i[qb] = (i << 1) | (flat ? 1 : 0);
-            } else if (flat) {
+            } else if (flat && !big) {
                 // rows in chunks of four pixels; a chunk's positions past the
                 // row's end repeat its last pixel (the same minimum again)
                 int32_t* row = tile + __mul24(py0 - db0, DW) + (px0 - dx0);
