@@ -283,6 +283,30 @@ def test_big_mesh_windows_configs(cfg):
             os.environ["QPSO_RF_CFG"] = old
 
 
+@pytest.mark.parametrize("mesh_kind", ["cached", "windowed"])
+def test_queue_overflow(scene1, mesh_kind):
+    # a big-face threshold of 1 pixel sends nearly every face to the wave
+    # queue: it overflows into the small-face queue, which overflows into
+    # the drawing lane (cached mesh: the 10k-face config1 mesh; windowed: the
+    # 16k-face mesh, whose queues are drawn at the end of each band)
+    if mesh_kind == "cached":
+        scene, params = scene1
+        cfg = 0
+    else:
+        mesh = qpso.mesh_uv_ellipsoid(103, 81, 70.0, 50.0, 35.0, 4, 8.0, 16.0, 5)
+        rng = np.random.default_rng(77)
+        pose = W.gt_pose(rng, xy=60.0, z=(700.0, 900.0), seed=77)
+        depth, labels = qpso.scene_render([(mesh, pose)], W.CAM, 1100.0, 2.0, 0.05, 78)
+        scene = W.Scene([(mesh, pose)], depth, labels)
+        params = qpso.make_params(16, 2, w=0.7, c1=1.5, c2=1.5, cam=W.CAM, seed=0)
+        cfg = 2
+    with qpso.Context(0) as c:
+        c.set_tuning(cfg, 1)
+        poses = sample_poses(scene, params.n_particles, seed=50, spread_mm=10.0, wild=0.2)
+        check_eval(c, scene, params, poses, frames=False)
+        check_eval(c, scene, params, poses)
+
+
 def test_big_mesh_without_runs(ctx):
     # the same 16k-face mesh with its faces shuffled: corners of neighbouring
     # faces are far apart in vertex order, so there are no vertex windows and
