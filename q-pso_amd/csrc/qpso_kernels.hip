@@ -708,10 +708,10 @@ __device__ __forceinline__ void rf_body(const QpsoSolveDev* __restrict__ solves,
     const bool use_mask = !runs && n_faces <= 64 * BT;
     // windows: bit w, window w of a band (the same windows every band) still
     // has faces reaching later bands; a window without is neither projected
-    // nor culled again.  wpar: the windows drawn so far mod 3, the slot of
+    // nor culled again.  wslot: the windows drawn so far mod 3, the slot of
     // s_later the current window uses.
     uint64_t wmask = ~0ull;
-    int wpar = 0;
+    int wslot = 0;
     for (int32_t b0 = ty0, b1; has_tile && b0 < ty1; b0 = b1) {
         b1 = band_end(b0);  // band rows [b0, b1)
         const int32_t nb = b1 - b0;
@@ -928,14 +928,14 @@ __device__ __forceinline__ void rf_body(const QpsoSolveDev* __restrict__ solves,
         // windows: the queued faces wait for the band's end (they are
         // projected again from the corner table there); every wave is done
         // with this window's cache before the next one is projected.  The
-        // window's flag goes to slot wpar, read after the barrier; the next
+        // window's flag goes to slot wslot, read after the barrier; the next
         // slot (last read two windows ago, before the previous barrier) is
         // cleared before it.
-        if (__ballot(fmask != 0ull) != 0ull && lane == 0) s_later[wpar] = 1;
-        if (tid == 0) s_later[wpar == 2 ? 0 : wpar + 1] = 0;
+        if (__ballot(fmask != 0ull) != 0ull && lane == 0) s_later[wslot] = 1;
+        if (tid == 0) s_later[wslot == 2 ? 0 : wslot + 1] = 0;
         __syncthreads();
-        if (wi < 64 && s_later[wpar] == 0) wmask &= ~(1ull << wi);
-        wpar = wpar == 2 ? 0 : wpar + 1;
+        if (wi < 64 && s_later[wslot] == 0) wmask &= ~(1ull << wi);
+        wslot = wslot == 2 ? 0 : wslot + 1;
         if (run >= S.n_runs || S.runs[3 * run] >= Fe) break;
         }
         __syncthreads();
